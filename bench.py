@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark: D3Q19 Smagorinsky LES lid-driven cavity, 256^3 FP32, as 2x2x2
+subdomains of 128^3 on one GPU with on-device ghost-layer exchange.
+
+One step = one half of the AA pattern (alpha or beta kernel over all eight
+subdomains) plus the ghost exchange that follows it, as in
+CController::computeNextStep (src/CController.hpp:385-391).  MLUPS counts
+the configured 256^3 cells per step (src/CController.hpp:446-452).
+
+Prints one JSON line (rank 0).  The roofline entry divides the algorithmic
+152 B per cell update (19 loads + 19 stores of FP32) by the measured average
+duration of the step kernel (events on the group's stream).  The CPU baseline
+times the C oracle (OpenMP) on a bounded number of steps of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS & % HBM roofline (D3Q19 LES 256³ FP32); batched cases/s + p99 latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = 152.0
+
+WORKLOAD = dict(domain=(256, 256, 256), nums=(2, 2, 2), cs=0.17, lid=0.1, tau=0.5005, amp=1e-4,
+                seed=20260101)
+
+
+def build_group(exchange_mode=0, graph=True, wg=256, plane_padding=-1):
+    import numpy as np
+
+    from turbulent_lbm_multigpu_amd import _capi
+    from turbulent_lbm_multigpu_amd import decomposition as dec
+    from turbulent_lbm_multigpu_amd.solver import LbmGroup, LbmSolver
+
+    w = WORKLOAD
+    mono = dec.monolithic_size(w["domain"], w["nums"])
+    subs = dec.decompose(w["domain"], w["nums"])
+    solvers = []
+    for sd in subs:
+        bc = [sd.bc[0][0], sd.bc[0][1], sd.bc[1][0], sd.bc[1][1], sd.bc[2][0], sd.bc[2][1]]
+        s = LbmSolver(sd.domain.size, bc, inv_tau=1.0 / w["tau"], tau=w["tau"], smagorinsky_cs=w["cs"],
+                      lid_velocity=w["lid"], work_group_size=wg, plane_padding=plane_padding)
+        s.init_perturbed(w["amp"], w["seed"], 0, dec.key_origin(sd), mono)
+        if sd.has_lid:
+            o, sz = dec.lid_region(sd.domain.size)
+            s.set(_capi.FIELD_FLAGS, np.full(sz[::-1], _capi.FLAG_VELOCITY_INJECTION, np.int32), o, sz)
+        solvers.append(s)
+    grp = LbmGroup(solvers)
+    for sd in subs:
+        for c in sd.comms:
+            grp.add_comm(sd.uid, c.dst_id, c.send_origin, c.recv_origin, c.send_size, c.comm_direction)
+    grp.set_exchange_mode(exchange_mode)
+    grp.set_graph(graph)
+    return grp
+
+
+def cpu_baseline(budget_s=12.0):
+    """The C oracle (OpenMP, all host threads) on the same workload: build the
+    2x2x2 decomposition, run steps until the budget is spent."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    from turbulent_lbm_multigpu_amd import decomposition as dec
+
+    w = WORKLOAD
+    mono = dec.monolithic_size(w["domain"], w["nums"])
+    subs = []
+    for sd in dec.decompose(w["domain"], w["nums"]):
+        bc = [sd.bc[0][0], sd.bc[0][1], sd.bc[1][0], sd.bc[1][1], sd.bc[2][0], sd.bc[2][1]]
+        d = orc.OracleDomain(sd.domain.size, bc, inv_tau=1.0 / w["tau"], tau=w["tau"], smagorinsky_cs=w["cs"],
+                             lid_velocity=w["lid"], store_velocity=False, store_density=False)
+        d.reset(amplitude=w["amp"], seed=w["seed"], key_origin=dec.key_origin(sd), key_dims=mono)
+        if sd.has_lid:
+            o, sz = dec.lid_region(sd.domain.size)
+            d.set_flags(o, sz, orc.FLAG_INJECTION)
+        subs.append({"domain": d, "comms": [(c.dst_id, c.send_origin, c.recv_origin, c.send_size,
+                                             c.comm_direction) for c in sd.comms]})
+    multi = orc.OracleMulti(subs)
+    multi.run(2)  # warm-up
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        multi.run(2)
+        steps += 2
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    cells = float(np.prod(w["domain"]))
+    return {
+        "value": round(cells * steps / el / 1e6, 2),
+        "unit": "MLUPS",
+        "cores": int(orc.load().oracle_num_threads()),
+        "kind": "port",
+        "sample": f"C oracle (OpenMP), 256^3 LES as 2x2x2 subdomains, {steps} steps after 2 warm-up steps, "
+                  f"{el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--timing-steps", type=int, default=200)
+    ap.add_argument("--exchange-mode", type=int, default=0)
+    ap.add_argument("--wg", type=int, default=256)
+    ap.add_argument("--plane-padding", type=int, default=-1)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs every subdomain on a single device "
+              "(no multi-GPU transport); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from turbulent_lbm_multigpu_amd import _capi
+
+    lib = _capi.load_library()
+    print(f"[bench] building 256^3 LES 2x2x2 (exchange_mode={args.exchange_mode}, wg={args.wg})",
+          file=sys.stderr, flush=True)
+    grp = build_group(args.exchange_mode, not args.no_graph, args.wg, args.plane_padding)
+    grp.step(args.warmup)
+    grp.wait()
+    lib.lbm_device_synchronize(0)
+    t0 = time.perf_counter()
+    grp.step(args.steps)
+    grp.wait()
+    lib.lbm_device_synchronize(0)
+    t1 = time.perf_counter()
+    el = t1 - t0
+    cells = 256 ** 3
+    mlups = cells * args.steps / el / 1e6
+    print(f"[bench] timed {args.steps} steps in {el:.3f} s: {mlups:.0f} MLUPS", file=sys.stderr, flush=True)
+
+    # per-launch duration of the step kernel, events on the group's stream
+    grp.set_timing(True)
+    grp.step(args.timing_steps)
+    tm = grp.timing()
+    grp.set_timing(False)
+    kern_ms = tm["step_kernel_ms"]
+    achieved = BYTES_PER_CELL * cells / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
+    print(f"[bench] step kernel {kern_ms * 1e3:.1f} us/launch, exchange {tm['exchange_ms'] * 1e3:.1f} us/launch",
+          file=sys.stderr, flush=True)
+    mass, ke = grp.solvers[0].mass_energy()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_budget)
+        print(f"[bench] CPU baseline {cpu['value']} MLUPS on {cpu['cores']} threads", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "D3Q19 Smagorinsky LES lid-driven cavity 256^3 FP32, 2x2x2 subdomains of 128^3 "
+                        "with on-device ghost exchange",
+            "cs": WORKLOAD["cs"], "lid": WORKLOAD["lid"], "tau0": WORKLOAD["tau"],
+            "init": "f_eq(1, du), du ~ U(+-1e-4), seed 20260101",
+            "exchange_mode": args.exchange_mode, "work_group": args.wg, "graph": not args.no_graph,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "step_kernel_us": round(kern_ms * 1e3, 2),
+        "exchange_kernel_us": round(tm["exchange_ms"] * 1e3, 2),
+        "subdomain0_mass_ke": [mass, ke],
+    }
+    print(json.dumps(out), flush=True)
+    grp.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * lbm_capi.h -- C-ABI of the MI355X-native D3Q19 lattice-Boltzmann hot path.
+ *
+ * The entry points replace the device-facing surface of the reference's
+ * CLbmSolver<T> (src/CLbmSolver.hpp:67-1124) and the halo exchange of
+ * CController (src/CController.hpp:265-391).  Plain pointers and sizes only;
+ * every call returns an LBM_* status, and lbm_solver_error()/lbm_last_error()
+ * give the message (the reference accumulates the same text in a CError,
+ * src/lib/CError.hpp:42-80).  Host buffers are caller-owned; set* copies at
+ * call time and store* blocks until the data is on the host, as in the
+ * reference.  A solver, group or batch is not thread-safe (one host thread
+ * per object, as the reference's one command queue per solver).
+ *
+ * Layout: distributions are 19 planes, plane i holding population i for all
+ * cells in x-fastest order (x + X*(y + Y*z)); populations are numbered as in
+ * src/cl_programs/lbm_header.h:12-19.  Region buffers are dense
+ * [component][z][y][x] of the region size (src/CLbmSolver.hpp:695-957).
+ */
+#ifndef LBM_CAPI_H
+#define LBM_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_CAPI_VERSION 1
+
+/* status codes */
+#define LBM_OK 0
+#define LBM_ERR_INVALID_ARGUMENT 1
+#define LBM_ERR_OUT_OF_RANGE 2
+#define LBM_ERR_DEVICE 3
+#define LBM_ERR_OUT_OF_MEMORY 4
+#define LBM_ERR_PARAMETRIZATION 5
+#define LBM_ERR_DECOMPOSITION 6
+#define LBM_ERR_STATE 7
+#define LBM_ERR_CAPACITY 8
+
+/* cell flags: src/common.h:19-22, src/CLbmSolver.hpp:30-33 */
+#define LBM_FLAG_OBSTACLE 1
+#define LBM_FLAG_FLUID 2
+#define LBM_FLAG_VELOCITY_INJECTION 4
+#define LBM_FLAG_GHOST_LAYER 8
+
+/* storage of the distributions (compute type: f32 except LBM_STORAGE_F64) */
+#define LBM_STORAGE_F32 0
+#define LBM_STORAGE_F64 1
+#define LBM_STORAGE_F16_SHIFTED 2  /* stores f_i - w_i as IEEE half */
+#define LBM_STORAGE_BF16_SHIFTED 3 /* stores f_i - w_i as bfloat16 */
+
+/* forcing model */
+#define LBM_FORCING_REFERENCE 0 /* the alpha kernel's term chain, src/cl_programs/lbm_alpha.cl:173-298 */
+#define LBM_FORCING_BODY 1      /* f_i += 3 w_i e_i.(rho g) after collision, both kernels */
+
+/* fields of the region get/set calls */
+#define LBM_FIELD_DD 0       /* 19 components, compute type */
+#define LBM_FIELD_VELOCITY 1 /* 3 components, compute type */
+#define LBM_FIELD_DENSITY 2  /* 1 component, compute type */
+#define LBM_FIELD_FLAGS 3    /* 1 component, int32 */
+
+typedef struct lbm_solver lbm_solver;
+typedef struct lbm_group lbm_group;
+
+/* Construction parameters of one (sub)domain; lattice units throughout.
+ * Replaces the argument list of CLbmSolver::CLbmSolver
+ * (src/CLbmSolver.hpp:221-260) after CLbmSkeleton's parametrization
+ * (src/CLbmSkeleton.hpp:81-190), which the C++/Python host layers perform. */
+typedef struct lbm_solver_config {
+    int dims[3];            /* cells per axis, boundary and ghost layers included */
+    int bc[6];              /* flag of the x-, x+, y-, y+, z-, z+ faces (BC[3][2]) */
+    int storage;            /* LBM_STORAGE_* */
+    int forcing;            /* LBM_FORCING_* */
+    double inv_tau;         /* 1/tau of the BGK collision */
+    double tau;             /* tau0 of the Smagorinsky rate */
+    double smagorinsky_cs;  /* 0 gives plain BGK, bit for bit */
+    double gravitation[3];  /* lattice body force / gravitation */
+    double lid_velocity;    /* lattice lid speed (x) of VELOCITY_INJECTION cells */
+    int store_velocity;     /* kernels write the velocity field (STORE_VELOCITY) */
+    int store_density;      /* kernels write the density field (STORE_DENSITY) */
+    int work_group_size;    /* threads per work-group of the step kernels; 0 = 256 */
+    int plane_padding;      /* extra elements per distribution plane; -1 = default */
+    int device;             /* HIP device ordinal */
+} lbm_solver_config;
+
+/* ---- library ---------------------------------------------------------- */
+const char *lbm_version(void);
+/* message of the last failed call on this thread (creation failures included) */
+const char *lbm_last_error(void);
+int lbm_device_synchronize(int device);
+void lbm_solver_config_default(lbm_solver_config *cfg);
+
+/* ---- solver: CLbmSolver<T> (src/CLbmSolver.hpp:67-1124) ----------------- */
+/* constructor + reload() + reset() (src/CLbmSolver.hpp:221-260, 272-635) */
+int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out);
+int lbm_solver_destroy(lbm_solver *s);
+/* CError text of this solver (src/lib/CError.hpp:42-80) */
+const char *lbm_solver_error(const lbm_solver *s);
+/* reset(): init kernel, step counter 0 (src/CLbmSolver.hpp:619-635) */
+int lbm_solver_reset(lbm_solver *s);
+/* reset() with f = f_eq(1, du) on every cell off OBSTACLE faces, du drawn per
+ * component from a counter-based generator keyed by (seed, case_id, cell of
+ * the key grid); state written in the swapped layout (new; no reference) */
+int lbm_solver_init_perturbed(lbm_solver *s, double amplitude, uint64_t seed, uint64_t case_id,
+                              const int key_origin[3], const int key_dims[3]);
+/* simulationStep(): beta when the counter is even, alpha when odd
+ * (src/CLbmSolver.hpp:664-676); Alpha/Beta (src/CLbmSolver.hpp:637-659) */
+int lbm_solver_simulation_step(lbm_solver *s);
+int lbm_solver_simulation_step_alpha(lbm_solver *s);
+int lbm_solver_simulation_step_beta(lbm_solver *s);
+int lbm_solver_run(lbm_solver *s, int steps);
+/* wait(): clFinish (src/CLbmSolver.hpp:681-683) */
+int lbm_solver_wait(lbm_solver *s);
+int lbm_solver_get_step_counter(const lbm_solver *s, long long *out);
+int lbm_solver_set_step_counter(lbm_solver *s, long long counter);
+int lbm_solver_get_config(const lbm_solver *s, lbm_solver_config *out);
+/* kernel argument update of addDrivenCavityValue (src/CLbmSolver.hpp:262-270) */
+int lbm_solver_set_lid_velocity(lbm_solver *s, double lattice_lid);
+int lbm_solver_set_relaxation(lbm_solver *s, double inv_tau, double tau, double smagorinsky_cs);
+/* storeDensityDistribution / storeVelocity / storeDensity / storeFlags, full
+ * (origin == NULL) or region (src/CLbmSolver.hpp:688-717, 764-795, 827-870, 907-945) */
+int lbm_solver_store(lbm_solver *s, int field, const int origin[3], const int size[3], void *host_dst);
+/* setDensityDistribution (with optional norm filter), setVelocity,
+ * setDensity, setFlags (src/CLbmSolver.hpp:719-757, 804-821, 879-905, 947-978);
+ * norm != NULL copies only populations with norm . e_f > 0 (LBM_FIELD_DD) */
+int lbm_solver_set(lbm_solver *s, int field, const int origin[3], const int size[3], const void *host_src,
+                   const int norm[3]);
+/* getVelocityChecksum (src/CLbmSolver.hpp:1103-1123) */
+int lbm_solver_velocity_checksum(lbm_solver *s, float *out);
+/* FP64 reductions over FLUID cells of rho and kinetic energy 1/2 |j|^2/rho,
+ * moments taken from the distributions in the current step parity (new) */
+int lbm_solver_mass_energy(lbm_solver *s, double *mass, double *kinetic_energy);
+/* rho and j = sum f_i e_i of a region, from the distributions in the current
+ * step parity, in the compute type: rho[size], j[3][size] (new) */
+int lbm_solver_moments(lbm_solver *s, const int origin[3], const int size[3], void *host_rho, void *host_j);
+/* device-to-device box copy of distributions between two solvers of the same
+ * storage (one CComm transfer without the host round trip of
+ * src/CController.hpp:265-383); norm == NULL copies all 19 populations */
+int lbm_copy_box(lbm_solver *src, const int src_origin[3], lbm_solver *dst, const int dst_origin[3],
+                 const int size[3], const int norm[3]);
+/* raw device pointers (for callers that bring their own kernels) */
+int lbm_solver_device_pointers(lbm_solver *s, void **dd, void **flags, long long *plane_stride);
+
+/* ---- group: N subdomains stepped together on one device ----------------- */
+/* One fused step kernel over all member solvers per half-step, then the
+ * ghost-layer exchange of CController::syncAlpha/syncBeta
+ * (src/CController.hpp:265-391) as on-device copies, x faces, then y, then z. */
+int lbm_group_create(lbm_solver *const *solvers, int n, lbm_group **out);
+int lbm_group_destroy(lbm_group *g);
+const char *lbm_group_error(const lbm_group *g);
+/* CController::addCommunication(new CComm(...)) (src/CController.hpp:524-526,
+ * src/CComm.hpp:8-79) for member `member`: dst_member is the neighbour */
+int lbm_group_add_comm(lbm_group *g, int member, int dst_member, const int send_origin[3],
+                       const int recv_origin[3], const int size[3], const int normal[3]);
+/* exchange mode: 0 = all 19 populations after alpha (reference),
+ * 1 = only the 5 populations an inner cell reads from a ghost */
+int lbm_group_set_exchange_mode(lbm_group *g, int mode);
+/* replay the step pair as a hipGraph (1) or launch eagerly (0) */
+int lbm_group_set_graph(lbm_group *g, int enable);
+/* computeNextStep() for all members, `steps` times (src/CController.hpp:385-391) */
+int lbm_group_step(lbm_group *g, int steps);
+/* one exchange on its own: syncAlpha (after_beta = 0) or syncBeta
+ * (after_beta = 1) of every member (src/CController.hpp:265-383) */
+int lbm_group_sync(lbm_group *g, int after_beta);
+int lbm_group_wait(lbm_group *g);
+/* average device time per launch of the step kernels and the exchange kernels
+ * since the last reset, measured with events on the group's stream when
+ * timing is on */
+int lbm_group_set_timing(lbm_group *g, int enable);
+int lbm_group_get_timing(lbm_group *g, double *step_kernel_ms, long long *step_kernel_launches,
+                         double *exchange_ms, long long *exchange_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_CAPI_H */

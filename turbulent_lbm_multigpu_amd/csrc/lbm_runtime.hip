@@ -1,0 +1,1221 @@
+// lbm_runtime.hip -- host runtime and auxiliary kernels behind include/lbm_capi.h.
+//
+// Objects:
+//   lbm_solver  one (sub)domain: CLbmSolver<T> (src/CLbmSolver.hpp:67-1124)
+//   lbm_group   N solvers stepped by one fused kernel per half-step plus the
+//               ghost-layer exchange of CController::syncAlpha/syncBeta
+//               (src/CController.hpp:265-391) as on-device copies.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_capi.h"
+#include "lbm_device.hpp"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int record(std::string *slot, int code, const std::string &msg) {
+    if (slot) *slot = msg;
+    g_last_error = msg;
+    return code;
+}
+
+struct Box {
+    int o[3];
+    int s[3];
+};
+
+// ---- auxiliary kernels ------------------------------------------------------
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double unit01(uint64_t seed, uint64_t case_id, uint64_t key, int comp) {
+    uint64_t h = mix64(seed ^ mix64(case_id ^ mix64(key * 4ull + (uint64_t)comp)));
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+struct InitArgs {
+    int bc[6];
+    double amp;
+    uint64_t seed, case_id;
+    int key_origin[3];
+    int key_dims[3];
+};
+
+// init_kernel (src/cl_programs/lbm_init.cl:32-237) + seeded perturbation.
+template <int ST>
+__global__ void init_kernel(DomDev D, InitArgs a) {
+    using SP = Storage<ST>;
+    using E = typename SP::E;
+    using T = typename SP::T;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= D.cells) return;
+    const int x = (int)(g % D.nx), y = (int)((g / D.nx) % D.ny), z = (int)(g / ((long long)D.nx * D.ny));
+    int flag = kFlagFluid;
+    if (x == 0) flag = a.bc[0];
+    else if (x == D.nx - 1) flag = a.bc[1];
+    else if (y == 0) flag = a.bc[2];
+    else if (y == D.ny - 1) flag = a.bc[3];
+    else if (z == 0) flag = a.bc[4];
+    else if (z == D.nz - 1) flag = a.bc[5];
+    D.flags[g] = (uint8_t)flag;
+    const bool outer = (x == 0 && a.bc[0] == kFlagObstacle) || (x == D.nx - 1 && a.bc[1] == kFlagObstacle) ||
+                       (y == 0 && a.bc[2] == kFlagObstacle) || (y == D.ny - 1 && a.bc[3] == kFlagObstacle) ||
+                       (z == 0 && a.bc[4] == kFlagObstacle) || (z == D.nz - 1 && a.bc[5] == kFlagObstacle);
+    T u[3] = {T(0), T(0), T(0)};
+    if (a.amp != 0.0 && !outer) {
+        const long long kx = a.key_origin[0] + x, ky = a.key_origin[1] + y, kz = a.key_origin[2] + z;
+        const uint64_t key = (uint64_t)(kx + (long long)a.key_dims[0] * (ky + (long long)a.key_dims[1] * kz));
+        for (int c = 0; c < 3; ++c) u[c] = (T)(a.amp * (2.0 * unit01(a.seed, a.case_id, key, c) - 1.0));
+    }
+    const T one = T(1);
+    const T dd_param = one - T(1.5) * vel2_of(u[0], u[1], u[2]);
+    T f[19];
+    feq19(dd_param, u[0], u[1], u[2], f);
+    E *dd = (E *)D.dd;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) SP::store(dd + (size_t)opp(i) * D.plane + g, opp(i), f[i]);
+    if (D.vel) {
+        T *v = (T *)D.vel;
+        v[g] = u[0];
+        v[D.cells + g] = u[1];
+        v[2 * D.cells + g] = u[2];
+    }
+    if (D.dens) ((T *)D.dens)[g] = one;
+}
+
+__device__ __forceinline__ long long box_cell(const DomDev &D, const Box &b, long long e) {
+    const int bx = (int)(e % b.s[0]);
+    const int by = (int)((e / b.s[0]) % b.s[1]);
+    const int bz = (int)(e / ((long long)b.s[0] * b.s[1]));
+    return (long long)(b.o[0] + bx) + (long long)D.nx * ((b.o[1] + by) + (long long)D.ny * (b.o[2] + bz));
+}
+
+// Region -> dense [component][box] buffer (copy_buffer_rect,
+// src/cl_programs/copy_buffer_rect.cl:12-52, all components in one launch).
+template <int ST>
+__global__ void pack_kernel(DomDev D, int field, Box b, void *out) {
+    using SP = Storage<ST>;
+    using E = typename SP::E;
+    using T = typename SP::T;
+    const long long n = (long long)b.s[0] * b.s[1] * b.s[2];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const long long g = box_cell(D, b, e);
+    if (field == LBM_FIELD_DD) {
+        const E *dd = (const E *)D.dd;
+        T *o = (T *)out;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) o[i * n + e] = SP::load(dd + (size_t)i * D.plane + g, i);
+    } else if (field == LBM_FIELD_VELOCITY) {
+        const T *v = (const T *)D.vel;
+        T *o = (T *)out;
+        for (int c = 0; c < 3; ++c) o[c * n + e] = v[c * D.cells + g];
+    } else if (field == LBM_FIELD_DENSITY) {
+        ((T *)out)[e] = ((const T *)D.dens)[g];
+    } else {
+        ((int32_t *)out)[e] = (int32_t)D.flags[g];
+    }
+}
+
+// Dense buffer -> region; for distributions only populations in `mask`.
+template <int ST>
+__global__ void unpack_kernel(DomDev D, int field, Box b, const void *in, uint32_t mask) {
+    using SP = Storage<ST>;
+    using E = typename SP::E;
+    using T = typename SP::T;
+    const long long n = (long long)b.s[0] * b.s[1] * b.s[2];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const long long g = box_cell(D, b, e);
+    if (field == LBM_FIELD_DD) {
+        E *dd = (E *)D.dd;
+        const T *s = (const T *)in;
+#pragma unroll
+        for (int i = 0; i < 19; ++i)
+            if (mask & (1u << i)) SP::store(dd + (size_t)i * D.plane + g, i, s[i * n + e]);
+    } else if (field == LBM_FIELD_VELOCITY) {
+        T *v = (T *)D.vel;
+        const T *s = (const T *)in;
+        for (int c = 0; c < 3; ++c) v[c * D.cells + g] = s[c * n + e];
+    } else if (field == LBM_FIELD_DENSITY) {
+        ((T *)D.dens)[g] = ((const T *)in)[e];
+    } else {
+        D.flags[g] = (uint8_t)((const int32_t *)in)[e];
+    }
+}
+
+// One ghost-layer transfer: a box of raw distribution elements from one
+// subdomain into another (populations in `mask`).
+struct CopyOp {
+    const void *src;
+    void *dst;
+    long long src_plane, dst_plane;
+    int src_nx, src_ny, dst_nx, dst_ny;
+    int so[3], dof[3], sz[3];
+    uint32_t mask;
+    int block_begin;
+    int cells;
+};
+
+template <typename E>
+__global__ void exchange_kernel(const CopyOp *__restrict__ ops, int nops) {
+    int k = 0;
+    while (k + 1 < nops && ops[k + 1].block_begin <= (int)blockIdx.x) ++k;
+    const CopyOp &op = ops[k];
+    const int e = ((int)blockIdx.x - op.block_begin) * (int)blockDim.x + (int)threadIdx.x;
+    if (e >= op.cells) return;
+    const int bx = e % op.sz[0];
+    const int by = (e / op.sz[0]) % op.sz[1];
+    const int bz = e / (op.sz[0] * op.sz[1]);
+    const long long gs = (long long)(op.so[0] + bx) + (long long)op.src_nx * ((op.so[1] + by) + (long long)op.src_ny * (op.so[2] + bz));
+    const long long gd = (long long)(op.dof[0] + bx) + (long long)op.dst_nx * ((op.dof[1] + by) + (long long)op.dst_ny * (op.dof[2] + bz));
+    const E *s = (const E *)op.src;
+    E *d = (E *)op.dst;
+#pragma unroll
+    for (int i = 0; i < 19; ++i)
+        if (op.mask & (1u << i)) d[(size_t)i * op.dst_plane + gd] = s[(size_t)i * op.src_plane + gs];
+}
+
+// rho and j = sum f_i e_i read in the current step parity: after an even
+// number of steps slot opp(i) holds f_i, after an odd number slot i does.
+template <int ST>
+__device__ __forceinline__ void cell_moments(const DomDev &D, long long g, bool swapped, double &r, double &a,
+                                             double &bb, double &c) {
+    using SP = Storage<ST>;
+    using E = typename SP::E;
+    const E *dd = (const E *)D.dd;
+    r = a = bb = c = 0.0;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const int slot = swapped ? opp(i) : i;
+        const double v = (double)SP::load(dd + (size_t)slot * D.plane + g, slot);
+        r += v;
+        a += v * ex(i);
+        bb += v * ey(i);
+        c += v * ez(i);
+    }
+}
+
+template <int ST>
+__global__ void moments_kernel(DomDev D, Box b, int swapped, void *rho_out, void *j_out) {
+    using T = typename Storage<ST>::T;
+    const long long n = (long long)b.s[0] * b.s[1] * b.s[2];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const long long g = box_cell(D, b, e);
+    double r, a, bb, c;
+    cell_moments<ST>(D, g, swapped != 0, r, a, bb, c);
+    ((T *)rho_out)[e] = (T)r;
+    T *j = (T *)j_out;
+    j[e] = (T)a;
+    j[n + e] = (T)bb;
+    j[2 * n + e] = (T)c;
+}
+
+// FP64 mass and kinetic energy over FLUID cells, first pass: one partial
+// pair per block (wavefront shuffles, then LDS across the block's waves).
+template <int ST>
+__global__ void __launch_bounds__(256) mass_energy_kernel(DomDev D, int swapped, double *partials) {
+    __shared__ double sm[2][4];
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0, k = 0.0;
+    if (g < D.cells && D.flags[g] == kFlagFluid) {
+        double r, a, bb, c;
+        cell_moments<ST>(D, g, swapped != 0, r, a, bb, c);
+        m = r;
+        k = 0.5 * (a * a + bb * bb + c * c) / r;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        m += __shfl_down(m, off, 64);
+        k += __shfl_down(k, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sm[0][wave] = m;
+        sm[1][wave] = k;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0, kk = 0.0;
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nw; ++w) {
+            mm += sm[0][w];
+            kk += sm[1][w];
+        }
+        partials[2 * blockIdx.x] = mm;
+        partials[2 * blockIdx.x + 1] = kk;
+    }
+}
+
+// Second pass: one block sums the partials in a fixed order (deterministic).
+__global__ void __launch_bounds__(256) sum_pairs_kernel(const double *partials, int n, double *out) {
+    __shared__ double sm[2][256];
+    double m = 0.0, k = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        m += partials[2 * i];
+        k += partials[2 * i + 1];
+    }
+    sm[0][threadIdx.x] = m;
+    sm[1][threadIdx.x] = k;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + s];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sm[0][0];
+        out[1] = sm[1][0];
+    }
+}
+
+size_t elem_size(int storage) {
+    switch (storage) {
+        case LBM_STORAGE_F32: return 4;
+        case LBM_STORAGE_F64: return 8;
+        default: return 2;
+    }
+}
+size_t compute_size(int storage) { return storage == LBM_STORAGE_F64 ? 8 : 4; }
+
+uint32_t norm_mask(const int *norm) {
+    if (!norm) return (1u << 19) - 1;
+    uint32_t m = 0;
+    for (int i = 0; i < 19; ++i)
+        if (norm[0] * ex(i) + norm[1] * ey(i) + norm[2] * ez(i) > 0) m |= 1u << i;
+    return m;
+}
+
+}  // namespace
+
+// ---- objects ----------------------------------------------------------------
+
+struct lbm_solver {
+    lbm_solver_config cfg{};
+    long long cells = 0, plane = 0;
+    int wg = 256;
+    void *dd = nullptr;
+    uint8_t *flags = nullptr;
+    void *vel = nullptr;
+    void *dens = nullptr;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    long long step_counter = 0;
+    std::string err;
+    DomDev host_desc{};
+    DomDev *d_desc = nullptr;  // single-domain launch descriptor
+    int *d_tile = nullptr;
+    int ntiles = 0;
+    void *staging = nullptr;
+    size_t staging_bytes = 0;
+    double *d_partials = nullptr;
+    size_t partial_count = 0;
+    unsigned version = 0;      // bumps when kernel parameters change
+    lbm_group *group = nullptr;
+};
+
+struct GroupComm {
+    int member, dst;
+    int send_origin[3], recv_origin[3], size[3], normal[3];
+};
+
+struct lbm_group {
+    std::vector<lbm_solver *> members;
+    std::vector<hipStream_t> saved_streams;
+    hipStream_t stream = nullptr;
+    int storage = 0, wg = 256, device = 0;
+    std::string err;
+    DomDev *d_descs = nullptr;
+    int *d_tiles = nullptr;
+    int ntiles = 0;
+    std::vector<unsigned> seen_versions;
+    std::vector<GroupComm> comms;
+    bool plan_built = false;
+    int exchange_mode = 0;
+    // [0]=after alpha, [1]=after beta; per phase x/y/z
+    CopyOp *d_ops[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    int nops[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int nblocks[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    long long step_counter = 0;
+    bool use_graph = true;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    bool timing = false;
+    struct Ev {
+        hipEvent_t a, b;
+        int kind;
+    };
+    std::vector<Ev> events;
+    double acc_ms[2] = {0, 0};
+    long long acc_n[2] = {0, 0};
+};
+
+namespace {
+
+int fail(lbm_solver *s, int code, const std::string &msg) { return record(s ? &s->err : nullptr, code, msg); }
+int gfail(lbm_group *g, int code, const std::string &msg) { return record(g ? &g->err : nullptr, code, msg); }
+
+#define SCHK(expr)                                                                                         \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) return fail(s, LBM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define GCHK(expr)                                                                                          \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) return gfail(g, LBM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename F>
+int dispatch_storage(int storage, F &&f) {
+    switch (storage) {
+        case LBM_STORAGE_F32: return f(std::integral_constant<int, 0>());
+        case LBM_STORAGE_F64: return f(std::integral_constant<int, 1>());
+        case LBM_STORAGE_F16_SHIFTED: return f(std::integral_constant<int, 2>());
+        default: return f(std::integral_constant<int, 3>());
+    }
+}
+
+void fill_desc(lbm_solver *s) {
+    DomDev &d = s->host_desc;
+    d.dd = s->dd;
+    d.flags = s->flags;
+    d.vel = s->vel;
+    d.dens = s->dens;
+    d.plane = s->plane;
+    d.cells = s->cells;
+    d.nx = s->cfg.dims[0];
+    d.ny = s->cfg.dims[1];
+    d.nz = s->cfg.dims[2];
+    d.tile_begin = 0;
+    d.ntiles = s->ntiles;
+    d.forcing = s->cfg.forcing;
+    d.store_velocity = s->cfg.store_velocity && s->vel;
+    d.store_density = s->cfg.store_density && s->dens;
+    d.inv_tau = s->cfg.inv_tau;
+    d.tau0 = s->cfg.tau;
+    const double cs = s->cfg.smagorinsky_cs;
+    d.les_coeff = cs > 0.0 ? 18.0 * std::sqrt(2.0) * cs * cs : 0.0;
+    d.gx = s->cfg.gravitation[0];
+    d.gy = s->cfg.gravitation[1];
+    d.gz = s->cfg.gravitation[2];
+    d.lid = s->cfg.lid_velocity;
+}
+
+int upload_desc(lbm_solver *s) {
+    fill_desc(s);
+    SCHK(hipMemcpyAsync(s->d_desc, &s->host_desc, sizeof(DomDev), hipMemcpyHostToDevice, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    s->version++;
+    return LBM_OK;
+}
+
+int launch_step(lbm_solver *s, bool beta) {
+    const int storage = s->cfg.storage;
+    return dispatch_storage(storage, [&](auto st) -> int {
+        constexpr int ST = decltype(st)::value;
+        if (beta)
+            step_kernel<ST, true><<<s->ntiles, s->wg, 0, s->stream>>>(s->d_desc, s->d_tile, 1);
+        else
+            step_kernel<ST, false><<<s->ntiles, s->wg, 0, s->stream>>>(s->d_desc, s->d_tile, 1);
+        SCHK(hipGetLastError());
+        return LBM_OK;
+    });
+}
+
+int ensure_staging(lbm_solver *s, size_t bytes) {
+    if (s->staging_bytes >= bytes) return LBM_OK;
+    if (s->staging) SCHK(hipFree(s->staging));
+    s->staging = nullptr;
+    s->staging_bytes = 0;
+    if (hipMalloc(&s->staging, bytes) != hipSuccess)
+        return fail(s, LBM_ERR_OUT_OF_MEMORY, "staging buffer allocation of " + std::to_string(bytes) + " bytes failed");
+    s->staging_bytes = bytes;
+    return LBM_OK;
+}
+
+int check_box(lbm_solver *s, const int *origin, const int *size, Box &b) {
+    for (int a = 0; a < 3; ++a) {
+        b.o[a] = origin ? origin[a] : 0;
+        b.s[a] = origin ? size[a] : s->cfg.dims[a];
+        if (b.s[a] <= 0 || b.o[a] < 0 || b.o[a] + b.s[a] > s->cfg.dims[a])
+            return fail(s, LBM_ERR_OUT_OF_RANGE,
+                        "region origin " + std::to_string(b.o[a]) + " size " + std::to_string(b.s[a]) +
+                            " outside axis " + std::to_string(a) + " of extent " + std::to_string(s->cfg.dims[a]));
+    }
+    return LBM_OK;
+}
+
+int field_components(int field) {
+    return field == LBM_FIELD_DD ? 19 : field == LBM_FIELD_VELOCITY ? 3 : 1;
+}
+
+}  // namespace
+
+// ---- library -----------------------------------------------------------------
+
+extern "C" const char *lbm_version(void) { return "turbulent_lbm_multigpu_amd 0.1 (gfx950)"; }
+extern "C" const char *lbm_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int lbm_device_synchronize(int device) {
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return record(nullptr, LBM_ERR_DEVICE, std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+extern "C" void lbm_solver_config_default(lbm_solver_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->dims[0] = c->dims[1] = c->dims[2] = 32;
+    for (int i = 0; i < 6; ++i) c->bc[i] = LBM_FLAG_OBSTACLE;
+    c->storage = LBM_STORAGE_F32;
+    c->forcing = LBM_FORCING_REFERENCE;
+    c->tau = 0.953575;  // src/CLbmSkeleton.hpp:142
+    c->inv_tau = 1.0 / 0.953575;
+    c->work_group_size = 256;
+    c->plane_padding = -1;
+    c->device = 0;
+}
+
+// ---- solver ------------------------------------------------------------------
+
+extern "C" int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out) {
+    if (!cfg || !out) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "lbm_solver_create: null argument");
+    *out = nullptr;
+    const lbm_solver_config &c = *cfg;
+    for (int a = 0; a < 3; ++a)
+        if (c.dims[a] < 3)
+            return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "dims must be >= 3 on every axis");
+    const long long cells = (long long)c.dims[0] * c.dims[1] * c.dims[2];
+    if (cells > (1ll << 31) - (1ll << 20))
+        return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "a (sub)domain is limited to 2^31 - 2^20 cells");
+    for (int i = 0; i < 6; ++i)
+        if (c.bc[i] < 0 || c.bc[i] > 255)
+            return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "boundary flags must be in [0, 255]");
+    if (c.storage < 0 || c.storage > 3) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "unknown storage");
+    if (c.forcing != 0 && c.forcing != 1) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "unknown forcing");
+    int wg = c.work_group_size == 0 ? 256 : c.work_group_size;
+    if (wg < 64 || wg > 1024 || wg % 64 != 0)
+        return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "work_group_size must be a multiple of 64 in [64, 1024]");
+    if (!(c.inv_tau > 0.0) || !std::isfinite(c.inv_tau))
+        return record(nullptr, LBM_ERR_PARAMETRIZATION, "inv_tau must be positive and finite");
+    if (c.smagorinsky_cs < 0.0 || (c.smagorinsky_cs > 0.0 && !(c.tau > 0.5)))
+        return record(nullptr, LBM_ERR_PARAMETRIZATION, "LES needs smagorinsky_cs >= 0 and tau > 0.5");
+
+    lbm_solver *s = new lbm_solver();
+    s->cfg = c;
+    s->cfg.work_group_size = wg;
+    s->wg = wg;
+    s->cells = cells;
+    const long long pad = c.plane_padding < 0 ? 0 : c.plane_padding;
+    s->plane = ((cells + 255) / 256) * 256 + pad;
+    s->ntiles = (int)((cells + wg - 1) / wg);
+    auto bail = [&](int code, const std::string &m) {
+        record(nullptr, code, m);
+        lbm_solver_destroy(s);
+        return code;
+    };
+    if (hipSetDevice(c.device) != hipSuccess) return bail(LBM_ERR_DEVICE, "hipSetDevice failed");
+    if (hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(LBM_ERR_DEVICE, "hipStreamCreate failed");
+    s->stream = s->own_stream;
+    const size_t es = elem_size(c.storage), cs = compute_size(c.storage);
+    if (hipMalloc(&s->dd, (size_t)19 * s->plane * es) != hipSuccess)
+        return bail(LBM_ERR_OUT_OF_MEMORY, "distribution buffer allocation failed");
+    if (hipMalloc((void **)&s->flags, (size_t)cells) != hipSuccess)
+        return bail(LBM_ERR_OUT_OF_MEMORY, "flag buffer allocation failed");
+    if (c.store_velocity && hipMalloc(&s->vel, (size_t)3 * cells * cs) != hipSuccess)
+        return bail(LBM_ERR_OUT_OF_MEMORY, "velocity buffer allocation failed");
+    if (c.store_density && hipMalloc(&s->dens, (size_t)cells * cs) != hipSuccess)
+        return bail(LBM_ERR_OUT_OF_MEMORY, "density buffer allocation failed");
+    if (hipMalloc((void **)&s->d_desc, sizeof(DomDev)) != hipSuccess ||
+        hipMalloc((void **)&s->d_tile, sizeof(int) * (size_t)s->ntiles) != hipSuccess)
+        return bail(LBM_ERR_OUT_OF_MEMORY, "descriptor allocation failed");
+    if (hipMemsetAsync(s->d_tile, 0, sizeof(int) * (size_t)s->ntiles, s->stream) != hipSuccess)
+        return bail(LBM_ERR_DEVICE, "tile table initialisation failed");
+    if (upload_desc(s) != LBM_OK) return bail(LBM_ERR_DEVICE, s->err);
+    int rc = lbm_solver_reset(s);
+    if (rc != LBM_OK) return bail(rc, s->err);
+    *out = s;
+    return LBM_OK;
+}
+
+extern "C" int lbm_solver_destroy(lbm_solver *s) {
+    if (!s) return LBM_OK;
+    if (s->group) return fail(s, LBM_ERR_STATE, "destroy the group before its member solvers");
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(s->dd);
+    (void)hipFree(s->flags);
+    (void)hipFree(s->vel);
+    (void)hipFree(s->dens);
+    (void)hipFree(s->d_desc);
+    (void)hipFree(s->d_tile);
+    (void)hipFree(s->staging);
+    (void)hipFree(s->d_partials);
+    if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+    delete s;
+    return LBM_OK;
+}
+
+extern "C" const char *lbm_solver_error(const lbm_solver *s) { return s ? s->err.c_str() : g_last_error.c_str(); }
+
+static int init_common(lbm_solver *s, const InitArgs &a) {
+    s->step_counter = 0;
+    return dispatch_storage(s->cfg.storage, [&](auto st) -> int {
+        constexpr int ST = decltype(st)::value;
+        const int blocks = (int)((s->cells + 255) / 256);
+        init_kernel<ST><<<blocks, 256, 0, s->stream>>>(s->host_desc, a);
+        SCHK(hipGetLastError());
+        SCHK(hipStreamSynchronize(s->stream));
+        return LBM_OK;
+    });
+}
+
+extern "C" int lbm_solver_reset(lbm_solver *s) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    InitArgs a{};
+    for (int i = 0; i < 6; ++i) a.bc[i] = s->cfg.bc[i];
+    a.amp = 0.0;
+    for (int i = 0; i < 3; ++i) a.key_dims[i] = s->cfg.dims[i];
+    return init_common(s, a);
+}
+
+extern "C" int lbm_solver_init_perturbed(lbm_solver *s, double amplitude, uint64_t seed, uint64_t case_id,
+                                         const int key_origin[3], const int key_dims[3]) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    if (!std::isfinite(amplitude)) return fail(s, LBM_ERR_INVALID_ARGUMENT, "amplitude must be finite");
+    InitArgs a{};
+    for (int i = 0; i < 6; ++i) a.bc[i] = s->cfg.bc[i];
+    a.amp = amplitude;
+    a.seed = seed;
+    a.case_id = case_id;
+    for (int i = 0; i < 3; ++i) {
+        a.key_origin[i] = key_origin ? key_origin[i] : 0;
+        a.key_dims[i] = key_dims ? key_dims[i] : s->cfg.dims[i];
+        if (a.key_dims[i] <= 0 || a.key_origin[i] < 0 || a.key_origin[i] + s->cfg.dims[i] > a.key_dims[i])
+            return fail(s, LBM_ERR_OUT_OF_RANGE, "key grid must contain the domain");
+    }
+    return init_common(s, a);
+}
+
+extern "C" int lbm_solver_simulation_step_alpha(lbm_solver *s) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    return launch_step(s, false);
+}
+extern "C" int lbm_solver_simulation_step_beta(lbm_solver *s) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    return launch_step(s, true);
+}
+extern "C" int lbm_solver_simulation_step(lbm_solver *s) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    int rc = launch_step(s, (s->step_counter & 1) == 0);
+    if (rc == LBM_OK) s->step_counter++;
+    return rc;
+}
+extern "C" int lbm_solver_run(lbm_solver *s, int steps) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    if (steps < 0) return fail(s, LBM_ERR_INVALID_ARGUMENT, "steps must be >= 0");
+    for (int i = 0; i < steps; ++i) {
+        int rc = lbm_solver_simulation_step(s);
+        if (rc) return rc;
+    }
+    return LBM_OK;
+}
+extern "C" int lbm_solver_wait(lbm_solver *s) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    SCHK(hipStreamSynchronize(s->stream));
+    return LBM_OK;
+}
+extern "C" int lbm_solver_get_step_counter(const lbm_solver *s, long long *out) {
+    if (!s || !out) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    *out = s->step_counter;
+    return LBM_OK;
+}
+extern "C" int lbm_solver_set_step_counter(lbm_solver *s, long long counter) {
+    if (!s || counter < 0) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad argument");
+    s->step_counter = counter;
+    return LBM_OK;
+}
+extern "C" int lbm_solver_get_config(const lbm_solver *s, lbm_solver_config *out) {
+    if (!s || !out) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    *out = s->cfg;
+    return LBM_OK;
+}
+extern "C" int lbm_solver_set_lid_velocity(lbm_solver *s, double lid) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    s->cfg.lid_velocity = lid;
+    return upload_desc(s);
+}
+extern "C" int lbm_solver_set_relaxation(lbm_solver *s, double inv_tau, double tau, double cs) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    if (!(inv_tau > 0.0) || cs < 0.0 || (cs > 0.0 && !(tau > 0.5)))
+        return fail(s, LBM_ERR_PARAMETRIZATION, "bad relaxation parameters");
+    s->cfg.inv_tau = inv_tau;
+    s->cfg.tau = tau;
+    s->cfg.smagorinsky_cs = cs;
+    return upload_desc(s);
+}
+
+extern "C" int lbm_solver_store(lbm_solver *s, int field, const int origin[3], const int size[3], void *host_dst) {
+    if (!s || !host_dst || (origin && !size)) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    if (field < 0 || field > 3) return fail(s, LBM_ERR_INVALID_ARGUMENT, "unknown field");
+    if (field == LBM_FIELD_VELOCITY && !s->vel)
+        return fail(s, LBM_ERR_STATE, "velocity is not stored by this solver (store_velocity = 0)");
+    if (field == LBM_FIELD_DENSITY && !s->dens)
+        return fail(s, LBM_ERR_STATE, "density is not stored by this solver (store_density = 0)");
+    Box b;
+    int rc = check_box(s, origin, size, b);
+    if (rc) return rc;
+    const long long n = (long long)b.s[0] * b.s[1] * b.s[2];
+    const size_t bytes = (size_t)n * field_components(field) * (field == LBM_FIELD_FLAGS ? 4 : compute_size(s->cfg.storage));
+    rc = ensure_staging(s, bytes);
+    if (rc) return rc;
+    rc = dispatch_storage(s->cfg.storage, [&](auto st) -> int {
+        constexpr int ST = decltype(st)::value;
+        pack_kernel<ST><<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(s->host_desc, field, b, s->staging);
+        SCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (rc) return rc;
+    SCHK(hipMemcpyAsync(host_dst, s->staging, bytes, hipMemcpyDeviceToHost, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    return LBM_OK;
+}
+
+extern "C" int lbm_solver_set(lbm_solver *s, int field, const int origin[3], const int size[3], const void *host_src,
+                              const int norm[3]) {
+    if (!s || !host_src || (origin && !size)) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    if (field < 0 || field > 3) return fail(s, LBM_ERR_INVALID_ARGUMENT, "unknown field");
+    if (norm && field != LBM_FIELD_DD) return fail(s, LBM_ERR_INVALID_ARGUMENT, "norm filter applies to distributions only");
+    if (field == LBM_FIELD_VELOCITY && !s->vel)
+        return fail(s, LBM_ERR_STATE, "velocity is not stored by this solver (store_velocity = 0)");
+    if (field == LBM_FIELD_DENSITY && !s->dens)
+        return fail(s, LBM_ERR_STATE, "density is not stored by this solver (store_density = 0)");
+    Box b;
+    int rc = check_box(s, origin, size, b);
+    if (rc) return rc;
+    const long long n = (long long)b.s[0] * b.s[1] * b.s[2];
+    if (field == LBM_FIELD_FLAGS) {
+        const int32_t *fl = (const int32_t *)host_src;
+        for (long long i = 0; i < n; ++i)
+            if (fl[i] < 0 || fl[i] > 255)
+                return fail(s, LBM_ERR_INVALID_ARGUMENT, "flag values must be in [0, 255]");
+    }
+    const size_t bytes = (size_t)n * field_components(field) * (field == LBM_FIELD_FLAGS ? 4 : compute_size(s->cfg.storage));
+    rc = ensure_staging(s, bytes);
+    if (rc) return rc;
+    SCHK(hipMemcpyAsync(s->staging, host_src, bytes, hipMemcpyHostToDevice, s->stream));
+    const uint32_t mask = norm_mask(norm);
+    rc = dispatch_storage(s->cfg.storage, [&](auto st) -> int {
+        constexpr int ST = decltype(st)::value;
+        unpack_kernel<ST><<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(s->host_desc, field, b, s->staging, mask);
+        SCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (rc) return rc;
+    SCHK(hipStreamSynchronize(s->stream));
+    return LBM_OK;
+}
+
+extern "C" int lbm_solver_velocity_checksum(lbm_solver *s, float *out) {
+    if (!s || !out) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    if (!s->vel) return fail(s, LBM_ERR_STATE, "velocity is not stored by this solver (store_velocity = 0)");
+    // src/CLbmSolver.hpp:1103-1123: float sum over FLUID cells, cell order
+    std::vector<int32_t> fl((size_t)s->cells);
+    int rc = lbm_solver_store(s, LBM_FIELD_FLAGS, nullptr, nullptr, fl.data());
+    if (rc) return rc;
+    float checksum = 0.0f;
+    if (s->cfg.storage == LBM_STORAGE_F64) {
+        std::vector<double> v((size_t)s->cells * 3);
+        rc = lbm_solver_store(s, LBM_FIELD_VELOCITY, nullptr, nullptr, v.data());
+        if (rc) return rc;
+        for (long long a = 0; a < s->cells; ++a)
+            if (fl[a] == LBM_FLAG_FLUID) checksum += (float)(v[a] + v[s->cells + a] + v[2 * s->cells + a]);
+    } else {
+        std::vector<float> v((size_t)s->cells * 3);
+        rc = lbm_solver_store(s, LBM_FIELD_VELOCITY, nullptr, nullptr, v.data());
+        if (rc) return rc;
+        for (long long a = 0; a < s->cells; ++a)
+            if (fl[a] == LBM_FLAG_FLUID) checksum += v[a] + v[s->cells + a] + v[2 * s->cells + a];
+    }
+    *out = checksum;
+    return LBM_OK;
+}
+
+extern "C" int lbm_solver_mass_energy(lbm_solver *s, double *mass, double *ke) {
+    if (!s || !mass || !ke) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    const int blocks = (int)((s->cells + 255) / 256);
+    if (s->partial_count < (size_t)blocks + 1) {
+        if (s->d_partials) SCHK(hipFree(s->d_partials));
+        s->d_partials = nullptr;
+        if (hipMalloc((void **)&s->d_partials, sizeof(double) * 2 * ((size_t)blocks + 1)) != hipSuccess)
+            return fail(s, LBM_ERR_OUT_OF_MEMORY, "reduction buffer allocation failed");
+        s->partial_count = (size_t)blocks + 1;
+    }
+    const int swapped = (s->step_counter % 2) == 0;
+    int rc = dispatch_storage(s->cfg.storage, [&](auto st) -> int {
+        constexpr int ST = decltype(st)::value;
+        mass_energy_kernel<ST><<<blocks, 256, 0, s->stream>>>(s->host_desc, swapped, s->d_partials + 2);
+        SCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (rc) return rc;
+    sum_pairs_kernel<<<1, 256, 0, s->stream>>>(s->d_partials + 2, blocks, s->d_partials);
+    SCHK(hipGetLastError());
+    double res[2];
+    SCHK(hipMemcpyAsync(res, s->d_partials, sizeof(res), hipMemcpyDeviceToHost, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    *mass = res[0];
+    *ke = res[1];
+    return LBM_OK;
+}
+
+extern "C" int lbm_solver_moments(lbm_solver *s, const int origin[3], const int size[3], void *host_rho, void *host_j) {
+    if (!s || !host_rho || !host_j || (origin && !size)) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    Box b;
+    int rc = check_box(s, origin, size, b);
+    if (rc) return rc;
+    const long long n = (long long)b.s[0] * b.s[1] * b.s[2];
+    const size_t cs = compute_size(s->cfg.storage);
+    rc = ensure_staging(s, (size_t)n * 4 * cs);
+    if (rc) return rc;
+    const int swapped = (s->step_counter % 2) == 0;
+    char *base = (char *)s->staging;
+    rc = dispatch_storage(s->cfg.storage, [&](auto st) -> int {
+        constexpr int ST = decltype(st)::value;
+        moments_kernel<ST><<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(s->host_desc, b, swapped, base,
+                                                                                 base + (size_t)n * cs);
+        SCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (rc) return rc;
+    SCHK(hipMemcpyAsync(host_rho, base, (size_t)n * cs, hipMemcpyDeviceToHost, s->stream));
+    SCHK(hipMemcpyAsync(host_j, base + (size_t)n * cs, (size_t)n * 3 * cs, hipMemcpyDeviceToHost, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    return LBM_OK;
+}
+
+extern "C" int lbm_solver_device_pointers(lbm_solver *s, void **dd, void **flags, long long *plane) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    if (dd) *dd = s->dd;
+    if (flags) *flags = s->flags;
+    if (plane) *plane = s->plane;
+    return LBM_OK;
+}
+
+static int fill_op(lbm_solver *src, const int *so, lbm_solver *dst, const int *dof, const int *size, uint32_t mask,
+                   CopyOp &op, std::string &why) {
+    for (int a = 0; a < 3; ++a) {
+        if (size[a] <= 0 || so[a] < 0 || dof[a] < 0 || so[a] + size[a] > src->cfg.dims[a] ||
+            dof[a] + size[a] > dst->cfg.dims[a]) {
+            why = "ghost-layer box outside a subdomain on axis " + std::to_string(a);
+            return LBM_ERR_OUT_OF_RANGE;
+        }
+    }
+    op.src = src->dd;
+    op.dst = dst->dd;
+    op.src_plane = src->plane;
+    op.dst_plane = dst->plane;
+    op.src_nx = src->cfg.dims[0];
+    op.src_ny = src->cfg.dims[1];
+    op.dst_nx = dst->cfg.dims[0];
+    op.dst_ny = dst->cfg.dims[1];
+    for (int a = 0; a < 3; ++a) {
+        op.so[a] = so[a];
+        op.dof[a] = dof[a];
+        op.sz[a] = size[a];
+    }
+    op.mask = mask;
+    op.cells = size[0] * size[1] * size[2];
+    op.block_begin = 0;
+    return LBM_OK;
+}
+
+template <typename E>
+static void launch_exchange(const CopyOp *d_ops, int nops, int nblocks, hipStream_t stream) {
+    exchange_kernel<E><<<nblocks, 256, 0, stream>>>(d_ops, nops);
+}
+
+static hipError_t exchange_launch(int storage, const CopyOp *d_ops, int nops, int nblocks, hipStream_t stream) {
+    switch (elem_size(storage)) {
+        case 8: launch_exchange<uint64_t>(d_ops, nops, nblocks, stream); break;
+        case 4: launch_exchange<uint32_t>(d_ops, nops, nblocks, stream); break;
+        default: launch_exchange<uint16_t>(d_ops, nops, nblocks, stream); break;
+    }
+    return hipGetLastError();
+}
+
+extern "C" int lbm_copy_box(lbm_solver *src, const int src_origin[3], lbm_solver *dst, const int dst_origin[3],
+                            const int size[3], const int norm[3]) {
+    if (!src || !dst || !src_origin || !dst_origin || !size)
+        return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    lbm_solver *s = dst;
+    if (src->cfg.storage != dst->cfg.storage) return fail(s, LBM_ERR_INVALID_ARGUMENT, "storage types differ");
+    CopyOp op;
+    std::string why;
+    int rc = fill_op(src, src_origin, dst, dst_origin, size, norm_mask(norm), op, why);
+    if (rc) return fail(s, rc, why);
+    SCHK(hipStreamSynchronize(src->stream));
+    CopyOp *d_op = nullptr;
+    SCHK(hipMalloc((void **)&d_op, sizeof(CopyOp)));
+    SCHK(hipMemcpyAsync(d_op, &op, sizeof(CopyOp), hipMemcpyHostToDevice, s->stream));
+    SCHK(exchange_launch(s->cfg.storage, d_op, 1, (op.cells + 255) / 256, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    SCHK(hipFree(d_op));
+    return LBM_OK;
+}
+
+// ---- group -----------------------------------------------------------------
+
+static void group_free_plan(lbm_group *g) {
+    for (int k = 0; k < 2; ++k)
+        for (int p = 0; p < 3; ++p) {
+            if (g->d_ops[k][p]) (void)hipFree(g->d_ops[k][p]);
+            g->d_ops[k][p] = nullptr;
+            g->nops[k][p] = 0;
+            g->nblocks[k][p] = 0;
+        }
+    if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->graph_exec = nullptr;
+    g->graph = nullptr;
+    g->plan_built = false;
+}
+
+static int group_upload_descs(lbm_group *g) {
+    std::vector<DomDev> descs;
+    int tile = 0;
+    for (auto *m : g->members) {
+        fill_desc(m);
+        DomDev d = m->host_desc;
+        d.tile_begin = tile;
+        descs.push_back(d);
+        tile += m->ntiles;
+    }
+    GCHK(hipMemcpyAsync(g->d_descs, descs.data(), sizeof(DomDev) * descs.size(), hipMemcpyHostToDevice, g->stream));
+    GCHK(hipStreamSynchronize(g->stream));
+    for (size_t i = 0; i < g->members.size(); ++i) g->seen_versions[i] = g->members[i]->version;
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_create(lbm_solver *const *solvers, int n, lbm_group **out) {
+    if (!solvers || n <= 0 || !out) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad group arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!solvers[i]) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null member solver");
+        if (solvers[i]->group) return record(nullptr, LBM_ERR_STATE, "a solver can belong to one group only");
+        if (solvers[i]->cfg.storage != solvers[0]->cfg.storage || solvers[i]->wg != solvers[0]->wg ||
+            solvers[i]->cfg.device != solvers[0]->cfg.device)
+            return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "group members need the same storage, work-group size and device");
+        for (int j = 0; j < i; ++j)
+            if (solvers[j] == solvers[i]) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "duplicate member solver");
+    }
+    lbm_group *g = new lbm_group();
+    g->storage = solvers[0]->cfg.storage;
+    g->wg = solvers[0]->wg;
+    g->device = solvers[0]->cfg.device;
+    for (int i = 0; i < n; ++i) {
+        g->members.push_back(solvers[i]);
+        g->ntiles += solvers[i]->ntiles;
+    }
+    g->seen_versions.assign(n, ~0u);
+    auto bail = [&](int code, const std::string &m) {
+        record(nullptr, code, m);
+        for (auto *mm : g->members) mm->group = nullptr;
+        lbm_group_destroy(g);
+        return code;
+    };
+    if (hipSetDevice(g->device) != hipSuccess) return bail(LBM_ERR_DEVICE, "hipSetDevice failed");
+    for (auto *m : g->members)
+        if (hipStreamSynchronize(m->stream) != hipSuccess) return bail(LBM_ERR_DEVICE, "member stream sync failed");
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(LBM_ERR_DEVICE, "hipStreamCreate failed");
+    if (hipMalloc((void **)&g->d_descs, sizeof(DomDev) * n) != hipSuccess ||
+        hipMalloc((void **)&g->d_tiles, sizeof(int) * (size_t)g->ntiles) != hipSuccess)
+        return bail(LBM_ERR_OUT_OF_MEMORY, "group table allocation failed");
+    std::vector<int> tiles((size_t)g->ntiles);
+    {
+        int t = 0;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < solvers[i]->ntiles; ++k) tiles[t++] = i;
+    }
+    if (hipMemcpy(g->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(LBM_ERR_DEVICE, "tile table upload failed");
+    g->step_counter = solvers[0]->step_counter;
+    for (auto *m : g->members) {
+        if (m->step_counter != g->step_counter) return bail(LBM_ERR_STATE, "member step counters differ");
+        g->saved_streams.push_back(m->stream);
+        m->stream = g->stream;
+        m->group = g;
+    }
+    if (group_upload_descs(g) != LBM_OK) return bail(LBM_ERR_DEVICE, g->err);
+    *out = g;
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_destroy(lbm_group *g) {
+    if (!g) return LBM_OK;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (auto &e : g->events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    group_free_plan(g);
+    for (size_t i = 0; i < g->members.size() && i < g->saved_streams.size(); ++i) {
+        g->members[i]->stream = g->saved_streams[i];
+        g->members[i]->group = nullptr;
+    }
+    (void)hipFree(g->d_descs);
+    (void)hipFree(g->d_tiles);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return LBM_OK;
+}
+
+extern "C" const char *lbm_group_error(const lbm_group *g) { return g ? g->err.c_str() : g_last_error.c_str(); }
+
+extern "C" int lbm_group_add_comm(lbm_group *g, int member, int dst_member, const int send_origin[3],
+                                  const int recv_origin[3], const int size[3], const int normal[3]) {
+    if (!g || !send_origin || !recv_origin || !size || !normal)
+        return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    const int n = (int)g->members.size();
+    if (member < 0 || member >= n || dst_member < 0 || dst_member >= n)
+        return gfail(g, LBM_ERR_INVALID_ARGUMENT, "member index out of range");
+    int nz = 0;
+    for (int a = 0; a < 3; ++a) nz += normal[a] != 0;
+    if (nz != 1 || std::abs(normal[0] + normal[1] + normal[2]) != 1)
+        return gfail(g, LBM_ERR_INVALID_ARGUMENT, "comm normal must be a unit axis vector");
+    GroupComm c;
+    c.member = member;
+    c.dst = dst_member;
+    for (int a = 0; a < 3; ++a) {
+        c.send_origin[a] = send_origin[a];
+        c.recv_origin[a] = recv_origin[a];
+        c.size[a] = size[a];
+        c.normal[a] = normal[a];
+    }
+    g->comms.push_back(c);
+    group_free_plan(g);
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_set_exchange_mode(lbm_group *g, int mode) {
+    if (!g || (mode != 0 && mode != 1)) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "exchange mode must be 0 or 1");
+    g->exchange_mode = mode;
+    group_free_plan(g);
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_set_graph(lbm_group *g, int enable) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    g->use_graph = enable != 0;
+    return LBM_OK;
+}
+
+static int build_plan(lbm_group *g) {
+    std::vector<CopyOp> ops[2][3];
+    for (const GroupComm &c : g->comms) {
+        const GroupComm *p = nullptr;
+        for (const GroupComm &q : g->comms)
+            if (q.member == c.dst && q.dst == c.member && q.normal[0] == -c.normal[0] && q.normal[1] == -c.normal[1] &&
+                q.normal[2] == -c.normal[2]) {
+                p = &q;
+                break;
+            }
+        if (!p)
+            return gfail(g, LBM_ERR_STATE,
+                         "comm of member " + std::to_string(c.member) + " toward " + std::to_string(c.dst) +
+                             " has no counterpart");
+        const int axis = c.normal[0] ? 0 : c.normal[1] ? 1 : 2;
+        lbm_solver *me = g->members[c.member], *nb = g->members[c.dst];
+        std::string why;
+        // after alpha: neighbour's inner layer -> my ghost layer (src/CController.hpp:265-320)
+        uint32_t amask = (1u << 19) - 1;
+        if (g->exchange_mode == 1) {
+            amask = 0;
+            for (int i = 0; i < 19; ++i)
+                if (c.normal[0] * ex(i) + c.normal[1] * ey(i) + c.normal[2] * ez(i) < 0) amask |= 1u << i;
+        }
+        CopyOp a;
+        int rc = fill_op(nb, p->send_origin, me, c.recv_origin, c.size, amask, a, why);
+        if (rc) return gfail(g, rc, why);
+        ops[0][axis].push_back(a);
+        // after beta: neighbour's ghost layer -> my inner layer, n.e > 0 only (src/CController.hpp:322-383)
+        CopyOp b;
+        rc = fill_op(nb, p->recv_origin, me, c.send_origin, c.size, norm_mask(c.normal), b, why);
+        if (rc) return gfail(g, rc, why);
+        ops[1][axis].push_back(b);
+    }
+    for (int k = 0; k < 2; ++k)
+        for (int p = 0; p < 3; ++p) {
+            int blocks = 0;
+            for (auto &op : ops[k][p]) {
+                op.block_begin = blocks;
+                blocks += (op.cells + 255) / 256;
+            }
+            g->nops[k][p] = (int)ops[k][p].size();
+            g->nblocks[k][p] = blocks;
+            if (ops[k][p].empty()) continue;
+            GCHK(hipMalloc((void **)&g->d_ops[k][p], sizeof(CopyOp) * ops[k][p].size()));
+            GCHK(hipMemcpy(g->d_ops[k][p], ops[k][p].data(), sizeof(CopyOp) * ops[k][p].size(), hipMemcpyHostToDevice));
+        }
+    g->plan_built = true;
+    return LBM_OK;
+}
+
+static int group_launch_half(lbm_group *g, bool beta, bool timed) {
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (timed) {
+        GCHK(hipEventCreate(&ea));
+        GCHK(hipEventCreate(&eb));
+        GCHK(hipEventRecord(ea, g->stream));
+    }
+    int rc = dispatch_storage(g->storage, [&](auto st) -> int {
+        constexpr int ST = decltype(st)::value;
+        if (beta)
+            step_kernel<ST, true><<<g->ntiles, g->wg, 0, g->stream>>>(g->d_descs, g->d_tiles, 1);
+        else
+            step_kernel<ST, false><<<g->ntiles, g->wg, 0, g->stream>>>(g->d_descs, g->d_tiles, 1);
+        GCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (rc) return rc;
+    if (timed) {
+        GCHK(hipEventRecord(eb, g->stream));
+        g->events.push_back({ea, eb, 0});
+    }
+    // the kernel just run was beta -> syncBeta, alpha -> syncAlpha (src/CController.hpp:385-391)
+    const int k = beta ? 1 : 0;
+    for (int p = 0; p < 3; ++p) {
+        if (!g->nops[k][p]) continue;
+        hipEvent_t xa = nullptr, xb = nullptr;
+        if (timed) {
+            GCHK(hipEventCreate(&xa));
+            GCHK(hipEventCreate(&xb));
+            GCHK(hipEventRecord(xa, g->stream));
+        }
+        GCHK(exchange_launch(g->storage, g->d_ops[k][p], g->nops[k][p], g->nblocks[k][p], g->stream));
+        if (timed) {
+            GCHK(hipEventRecord(xb, g->stream));
+            g->events.push_back({xa, xb, 1});
+        }
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_step(lbm_group *g, int steps) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    if (steps < 0) return gfail(g, LBM_ERR_INVALID_ARGUMENT, "steps must be >= 0");
+    GCHK(hipSetDevice(g->device));
+    bool stale = false;
+    for (size_t i = 0; i < g->members.size(); ++i) stale |= g->seen_versions[i] != g->members[i]->version;
+    if (stale) {
+        int rc = group_upload_descs(g);
+        if (rc) return rc;
+    }
+    if (!g->plan_built) {
+        int rc = build_plan(g);
+        if (rc) return rc;
+    }
+    int done = 0;
+    while (done < steps) {
+        const bool beta = (g->step_counter & 1) == 0;
+        if (g->use_graph && !g->timing && beta && steps - done >= 2) {
+            if (!g->graph_exec) {
+                GCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+                int rc1 = group_launch_half(g, true, false);
+                int rc2 = rc1 ? rc1 : group_launch_half(g, false, false);
+                hipGraph_t graph = nullptr;
+                hipError_t ec = hipStreamEndCapture(g->stream, &graph);
+                if (rc2) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return rc2;
+                }
+                GCHK(ec);
+                g->graph = graph;
+                GCHK(hipGraphInstantiate(&g->graph_exec, g->graph, nullptr, nullptr, 0));
+            }
+            GCHK(hipGraphLaunch(g->graph_exec, g->stream));
+            g->step_counter += 2;
+            done += 2;
+        } else {
+            int rc = group_launch_half(g, beta, g->timing);
+            if (rc) return rc;
+            g->step_counter += 1;
+            done += 1;
+        }
+    }
+    for (auto *m : g->members) m->step_counter = g->step_counter;
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_sync(lbm_group *g, int after_beta) {
+    if (!g || (after_beta != 0 && after_beta != 1)) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad argument");
+    GCHK(hipSetDevice(g->device));
+    if (!g->plan_built) {
+        int rc = build_plan(g);
+        if (rc) return rc;
+    }
+    for (int p = 0; p < 3; ++p) {
+        if (!g->nops[after_beta][p]) continue;
+        GCHK(exchange_launch(g->storage, g->d_ops[after_beta][p], g->nops[after_beta][p], g->nblocks[after_beta][p],
+                             g->stream));
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_wait(lbm_group *g) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    GCHK(hipStreamSynchronize(g->stream));
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_set_timing(lbm_group *g, int enable) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    GCHK(hipStreamSynchronize(g->stream));
+    for (auto &e : g->events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    g->events.clear();
+    g->acc_ms[0] = g->acc_ms[1] = 0;
+    g->acc_n[0] = g->acc_n[1] = 0;
+    g->timing = enable != 0;
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_get_timing(lbm_group *g, double *step_ms, long long *step_n, double *xch_ms, long long *xch_n) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    GCHK(hipStreamSynchronize(g->stream));
+    for (auto &e : g->events) {
+        float ms = 0.f;
+        GCHK(hipEventElapsedTime(&ms, e.a, e.b));
+        g->acc_ms[e.kind] += ms;
+        g->acc_n[e.kind] += 1;
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    g->events.clear();
+    if (step_ms) *step_ms = g->acc_n[0] ? g->acc_ms[0] / g->acc_n[0] : 0.0;
+    if (step_n) *step_n = g->acc_n[0];
+    if (xch_ms) *xch_ms = g->acc_n[1] ? g->acc_ms[1] / g->acc_n[1] : 0.0;
+    if (xch_n) *xch_n = g->acc_n[1];
+    return LBM_OK;
+}
