@@ -145,8 +145,13 @@ def main():
     grp.set_timing(False)
     kern_ms = tm["step_kernel_ms"]
     achieved = BYTES_PER_CELL * cells / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
-    print(f"[bench] step kernel {kern_ms * 1e3:.1f} us/launch, exchange {tm['exchange_ms'] * 1e3:.1f} us/launch",
+    print(f"[bench] step kernel {kern_ms * 1e3:.1f} us/launch (beta {tm['beta_kernel_ms'] * 1e3:.1f}, "
+          f"alpha {tm['alpha_kernel_ms'] * 1e3:.1f}), exchange {tm['exchange_ms'] * 1e3:.1f} us/launch",
           file=sys.stderr, flush=True)
+    import ctypes
+    gbs, pms = ctypes.c_double(), ctypes.c_double()
+    lib.lbm_device_copy_bandwidth(0, 1 << 30, 20, ctypes.byref(gbs), ctypes.byref(pms))
+    print(f"[bench] plain device copy of 1 GiB: {gbs.value:.0f} GB/s", file=sys.stderr, flush=True)
     mass, ke = grp.solvers[0].mass_energy()
 
     cpu = None
@@ -185,6 +190,9 @@ def main():
         },
         "cpu_baseline": cpu,
         "step_kernel_us": round(kern_ms * 1e3, 2),
+        "beta_kernel_us": round(tm["beta_kernel_ms"] * 1e3, 2),
+        "alpha_kernel_us": round(tm["alpha_kernel_ms"] * 1e3, 2),
+        "copy_probe_gbs": round(gbs.value, 1),
         "exchange_kernel_us": round(tm["exchange_ms"] * 1e3, 2),
         "subdomain0_mass_ke": [mass, ke],
     }

@@ -91,6 +91,11 @@ const char *lbm_version(void);
 const char *lbm_last_error(void);
 int lbm_device_synchronize(int device);
 void lbm_solver_config_default(lbm_solver_config *cfg);
+/* read+write bandwidth of a plain device copy of `bytes` (stream_copy_kernel),
+ * averaged over `reps` launches: the achievable-HBM reference of the roofline
+ * and the known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters
+ * (new; no reference counterpart) */
+int lbm_device_copy_bandwidth(int device, size_t bytes, int reps, double *gbs, double *kernel_ms);
 
 /* ---- solver: CLbmSolver<T> (src/CLbmSolver.hpp:67-1124) ----------------- */
 /* constructor + reload() + reset() (src/CLbmSolver.hpp:221-260, 272-635) */
@@ -171,6 +176,8 @@ int lbm_group_wait(lbm_group *g);
 int lbm_group_set_timing(lbm_group *g, int enable);
 int lbm_group_get_timing(lbm_group *g, double *step_kernel_ms, long long *step_kernel_launches,
                          double *exchange_ms, long long *exchange_launches);
+/* the same averages split into beta and alpha step kernels */
+int lbm_group_get_timing_detail(lbm_group *g, double *beta_ms, double *alpha_ms, double *exchange_ms);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,7 @@ FIELD_FLAGS = 3
 
 EXPORTED_SYMBOLS = (
     "lbm_version", "lbm_last_error", "lbm_device_synchronize", "lbm_solver_config_default",
+    "lbm_device_copy_bandwidth",
     "lbm_solver_create", "lbm_solver_destroy", "lbm_solver_error", "lbm_solver_reset",
     "lbm_solver_init_perturbed", "lbm_solver_simulation_step", "lbm_solver_simulation_step_alpha",
     "lbm_solver_simulation_step_beta", "lbm_solver_run", "lbm_solver_wait",
@@ -51,6 +52,7 @@ EXPORTED_SYMBOLS = (
     "lbm_solver_device_pointers", "lbm_group_create", "lbm_group_destroy", "lbm_group_error",
     "lbm_group_add_comm", "lbm_group_set_exchange_mode", "lbm_group_set_graph", "lbm_group_step",
     "lbm_group_wait", "lbm_group_set_timing", "lbm_group_get_timing", "lbm_group_sync",
+    "lbm_group_get_timing_detail",
 )
 
 
@@ -110,6 +112,7 @@ def load_library(path: str | None = None):
     sig("lbm_last_error", C.c_char_p)
     sig("lbm_device_synchronize", I, I)
     sig("lbm_solver_config_default", None, C.POINTER(SolverConfig))
+    sig("lbm_device_copy_bandwidth", I, I, C.c_size_t, I, C.POINTER(D), C.POINTER(D))
     sig("lbm_solver_create", I, C.POINTER(SolverConfig), C.POINTER(P))
     sig("lbm_solver_destroy", I, P)
     sig("lbm_solver_error", C.c_char_p, P)
@@ -141,6 +144,7 @@ def load_library(path: str | None = None):
     sig("lbm_group_step", I, P, I)
     sig("lbm_group_wait", I, P)
     sig("lbm_group_sync", I, P, I)
+    sig("lbm_group_get_timing_detail", I, P, C.POINTER(D), C.POINTER(D), C.POINTER(D))
     sig("lbm_group_set_timing", I, P, I)
     sig("lbm_group_get_timing", I, P, C.POINTER(D), C.POINTER(C.c_longlong), C.POINTER(D),
         C.POINTER(C.c_longlong))

@@ -65,35 +65,64 @@ __host__ __device__ constexpr T weight(int i) {
 
 // ---- storage of the distributions -------------------------------------------
 // 0: f32, 1: f64, 2: f16 holding f - w, 3: bf16 holding f - w.
+// Accesses go through address-space-1 (global) pointers: a uniform 64-bit
+// plane base in SGPRs plus a 32-bit per-lane byte offset, which the compiler
+// turns into `global_load/store ... v_off, s[base]` (no 64-bit VALU address
+// arithmetic, no flat instructions).
+typedef __attribute__((address_space(1))) char gchar;
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) unsigned short gushort;
+typedef __attribute__((address_space(1))) unsigned char guchar;
+
+__device__ __forceinline__ unsigned short bf16_bits_rne(float v) {
+    unsigned u = __float_as_uint(v);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
 template <int ST>
 struct Storage;
 template <>
 struct Storage<0> {
     using E = float;
     using T = float;
-    __device__ static T load(const E *p, int) { return *p; }
-    __device__ static void store(E *p, int, T v) { *p = v; }
+    static constexpr unsigned kBytes = 4;
+    __device__ static T ld(const gchar *base, unsigned off, int) { return *(const gfloat *)(base + off); }
+    __device__ static void st(gchar *base, unsigned off, int, T v) { *(gfloat *)(base + off) = v; }
 };
 template <>
 struct Storage<1> {
     using E = double;
     using T = double;
-    __device__ static T load(const E *p, int) { return *p; }
-    __device__ static void store(E *p, int, T v) { *p = v; }
+    static constexpr unsigned kBytes = 8;
+    __device__ static T ld(const gchar *base, unsigned off, int) { return *(const gdouble *)(base + off); }
+    __device__ static void st(gchar *base, unsigned off, int, T v) { *(gdouble *)(base + off) = v; }
 };
 template <>
 struct Storage<2> {
     using E = __half;
     using T = float;
-    __device__ static T load(const E *p, int i) { return __half2float(*p) + weight<float>(i); }
-    __device__ static void store(E *p, int i, T v) { *p = __float2half(v - weight<float>(i)); }
+    static constexpr unsigned kBytes = 2;
+    __device__ static T ld(const gchar *base, unsigned off, int i) {
+        return __half2float(__ushort_as_half(*(const gushort *)(base + off))) + weight<float>(i);
+    }
+    __device__ static void st(gchar *base, unsigned off, int i, T v) {
+        *(gushort *)(base + off) = __half_as_ushort(__float2half(v - weight<float>(i)));
+    }
 };
 template <>
 struct Storage<3> {
     using E = __hip_bfloat16;
     using T = float;
-    __device__ static T load(const E *p, int i) { return __bfloat162float(*p) + weight<float>(i); }
-    __device__ static void store(E *p, int i, T v) { *p = __float2bfloat16(v - weight<float>(i)); }
+    static constexpr unsigned kBytes = 2;
+    __device__ static T ld(const gchar *base, unsigned off, int i) {
+        return __uint_as_float((unsigned)(*(const gushort *)(base + off)) << 16) + weight<float>(i);
+    }
+    __device__ static void st(gchar *base, unsigned off, int i, T v) {
+        *(gushort *)(base + off) = bf16_bits_rne(v - weight<float>(i));
+    }
 };
 
 // ---- per-(sub)domain launch descriptor ---------------------------------------
@@ -271,37 +300,72 @@ __device__ __forceinline__ int wrap_idx(int a, int cells) {
 }
 
 // ---- the step kernel -------------------------------------------------------
+// Gather of the beta kernel: f_i = S[opp(i)][W(g - d_i)].  WRAP is false for
+// blocks whose cells are all at least max|d| away from both ends of the
+// domain, where the linear wrap W (src/cl_programs/wrap.h:112-127) is the
+// identity.
+template <class SP, bool WRAP>
+__device__ __forceinline__ void beta_gather(gchar *const *pl, unsigned g, int cells, const int *d,
+                                            typename SP::T *f) {
+#pragma unroll
+    for (int i = 0; i < 18; ++i) {
+        unsigned idx = WRAP ? (unsigned)wrap_idx((int)g - d[i], cells) : g - (unsigned)d[i];
+        f[i] = SP::ld(pl[opp(i)], idx * SP::kBytes, opp(i));
+    }
+    f[18] = SP::ld(pl[18], g * SP::kBytes, 18);
+}
+
+template <class SP, bool WRAP>
+__device__ __forceinline__ void beta_scatter(gchar *const *pl, unsigned g, int cells, const int *d,
+                                             const typename SP::T *f) {
+#pragma unroll
+    for (int i = 0; i < 18; ++i) {
+        unsigned idx = WRAP ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
+        SP::st(pl[i], idx * SP::kBytes, i, f[i]);
+    }
+    SP::st(pl[18], g * SP::kBytes, 18, f[18]);
+}
+
 template <int ST, bool BETA>
 __global__ void __launch_bounds__(1024) step_kernel(const DomDev *__restrict__ doms, const int *__restrict__ tile_dom,
                                                     int xcd_map) {
     using SP = Storage<ST>;
-    using E = typename SP::E;
     using T = typename SP::T;
     const int nb = (int)gridDim.x;
     const int t = xcd_map ? xcd_tile((int)blockIdx.x, nb) : (int)blockIdx.x;
     const DomDev *D = doms + tile_dom[t];
     const int cells = (int)D->cells;
-    const int g = (t - D->tile_begin) * (int)blockDim.x + (int)threadIdx.x;
-    if (g >= cells) return;
-    E *dd = (E *)D->dd;
-    const long long P = D->plane;
-    const int flag = D->flags[g];
-    if (!BETA && flag == kFlagGhost) return;  // lbm_alpha.cl:30-32
+    const int block_first = (t - D->tile_begin) * (int)blockDim.x;
+    const int gi = block_first + (int)threadIdx.x;
+    if (gi >= cells) return;
+    const unsigned g = (unsigned)gi;
+    gchar *pl[19];
+    {
+        gchar *base = (gchar *)D->dd;
+        const size_t pb = (size_t)D->plane * SP::kBytes;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) pl[i] = base + (size_t)i * pb;
+    }
+    const int flag = ((const guchar *)D->flags)[g];
 
     T f[19];
-    int nbr[19];
+    int d[19];
+    bool wrap = false;
     if (BETA) {
         const int X = D->nx, XY = D->nx * D->ny;
 #pragma unroll
-        for (int i = 0; i < 18; ++i) {
-            const int d = ex(i) + X * ey(i) + XY * ez(i);
-            nbr[i] = d;
-            f[i] = SP::load(dd + (size_t)opp(i) * P + wrap_idx(g - d, cells), opp(i));
-        }
-        f[18] = SP::load(dd + (size_t)18 * P + g, 18);
+        for (int i = 0; i < 19; ++i) d[i] = ex(i) + X * ey(i) + XY * ez(i);
+        const int dmax = 1 + X + XY;
+        wrap = block_first < dmax || block_first + (int)blockDim.x + dmax > cells;
+        if (wrap)
+            beta_gather<SP, true>(pl, g, cells, d, f);
+        else
+            beta_gather<SP, false>(pl, g, cells, d, f);
     } else {
+        // the 19 loads do not wait for the flag; a GHOST cell (lbm_alpha.cl:30-32)
+        // matches no branch below and stores nothing
 #pragma unroll
-        for (int i = 0; i < 19; ++i) f[i] = SP::load(dd + (size_t)i * P + g, i);
+        for (int i = 0; i < 19; ++i) f[i] = SP::ld(pl[i], g * SP::kBytes, i);
     }
 
     T rho, vx, vy, vz;
@@ -383,22 +447,24 @@ __global__ void __launch_bounds__(1024) step_kernel(const DomDev *__restrict__ d
 
     if (write_dd) {
         if (BETA) {
-#pragma unroll
-            for (int i = 0; i < 18; ++i) SP::store(dd + (size_t)i * P + wrap_idx(g + nbr[i], cells), i, f[i]);
-            SP::store(dd + (size_t)18 * P + g, 18, f[18]);
+            if (wrap)
+                beta_scatter<SP, true>(pl, g, cells, d, f);
+            else
+                beta_scatter<SP, false>(pl, g, cells, d, f);
         } else {
 #pragma unroll
-            for (int i = 0; i < 19; ++i) SP::store(dd + (size_t)opp(i) * P + g, opp(i), f[i]);
+            for (int i = 0; i < 19; ++i) SP::st(pl[opp(i)], g * SP::kBytes, opp(i), f[i]);
         }
     }
-    if (BETA && flag == kFlagGhost) return;  // lbm_beta.cl:787-788
+    if (flag == kFlagGhost) return;  // lbm_alpha.cl:30-32, lbm_beta.cl:787-788
     if (D->store_velocity) {
-        T *v = (T *)D->vel;
-        v[g] = vx;
-        v[(size_t)cells + g] = vy;
-        v[2 * (size_t)cells + g] = vz;
+        gchar *v = (gchar *)D->vel;
+        const unsigned cs = (unsigned)sizeof(T);
+        *(__attribute__((address_space(1))) T *)(v + g * cs) = vx;
+        *(__attribute__((address_space(1))) T *)(v + (size_t)cells * cs + g * cs) = vy;
+        *(__attribute__((address_space(1))) T *)(v + 2 * (size_t)cells * cs + g * cs) = vz;
     }
-    if (D->store_density) ((T *)D->dens)[g] = dens_out;
+    if (D->store_density) *(__attribute__((address_space(1))) T *)((gchar *)D->dens + g * (unsigned)sizeof(T)) = dens_out;
 }
 
 }  // namespace lbm

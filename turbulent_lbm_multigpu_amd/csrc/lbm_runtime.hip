@@ -47,6 +47,12 @@ __device__ __forceinline__ double unit01(uint64_t seed, uint64_t case_id, uint64
     return (double)(h >> 11) * (1.0 / 9007199254740992.0);
 }
 
+// plane base of population i (global address space) and byte offset of cell g
+template <class SP>
+__device__ __forceinline__ gchar *plane_ptr(const DomDev &D, int i) {
+    return (gchar *)D.dd + (size_t)i * (size_t)D.plane * SP::kBytes;
+}
+
 struct InitArgs {
     int bc[6];
     double amp;
@@ -59,7 +65,6 @@ struct InitArgs {
 template <int ST>
 __global__ void init_kernel(DomDev D, InitArgs a) {
     using SP = Storage<ST>;
-    using E = typename SP::E;
     using T = typename SP::T;
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= D.cells) return;
@@ -85,9 +90,8 @@ __global__ void init_kernel(DomDev D, InitArgs a) {
     const T dd_param = one - T(1.5) * vel2_of(u[0], u[1], u[2]);
     T f[19];
     feq19(dd_param, u[0], u[1], u[2], f);
-    E *dd = (E *)D.dd;
 #pragma unroll
-    for (int i = 0; i < 19; ++i) SP::store(dd + (size_t)opp(i) * D.plane + g, opp(i), f[i]);
+    for (int i = 0; i < 19; ++i) SP::st(plane_ptr<SP>(D, opp(i)), (unsigned)g * SP::kBytes, opp(i), f[i]);
     if (D.vel) {
         T *v = (T *)D.vel;
         v[g] = u[0];
@@ -109,17 +113,15 @@ __device__ __forceinline__ long long box_cell(const DomDev &D, const Box &b, lon
 template <int ST>
 __global__ void pack_kernel(DomDev D, int field, Box b, void *out) {
     using SP = Storage<ST>;
-    using E = typename SP::E;
     using T = typename SP::T;
     const long long n = (long long)b.s[0] * b.s[1] * b.s[2];
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const long long g = box_cell(D, b, e);
     if (field == LBM_FIELD_DD) {
-        const E *dd = (const E *)D.dd;
         T *o = (T *)out;
 #pragma unroll
-        for (int i = 0; i < 19; ++i) o[i * n + e] = SP::load(dd + (size_t)i * D.plane + g, i);
+        for (int i = 0; i < 19; ++i) o[i * n + e] = SP::ld(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i);
     } else if (field == LBM_FIELD_VELOCITY) {
         const T *v = (const T *)D.vel;
         T *o = (T *)out;
@@ -135,18 +137,16 @@ __global__ void pack_kernel(DomDev D, int field, Box b, void *out) {
 template <int ST>
 __global__ void unpack_kernel(DomDev D, int field, Box b, const void *in, uint32_t mask) {
     using SP = Storage<ST>;
-    using E = typename SP::E;
     using T = typename SP::T;
     const long long n = (long long)b.s[0] * b.s[1] * b.s[2];
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const long long g = box_cell(D, b, e);
     if (field == LBM_FIELD_DD) {
-        E *dd = (E *)D.dd;
         const T *s = (const T *)in;
 #pragma unroll
         for (int i = 0; i < 19; ++i)
-            if (mask & (1u << i)) SP::store(dd + (size_t)i * D.plane + g, i, s[i * n + e]);
+            if (mask & (1u << i)) SP::st(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i, s[i * n + e]);
     } else if (field == LBM_FIELD_VELOCITY) {
         T *v = (T *)D.vel;
         const T *s = (const T *)in;
@@ -159,35 +159,48 @@ __global__ void unpack_kernel(DomDev D, int field, Box b, const void *in, uint32
 }
 
 // One ghost-layer transfer: a box of raw distribution elements from one
-// subdomain into another (populations in `mask`).
+// subdomain into another (populations listed in `pops`).  One work-item moves
+// one element: consecutive work-items walk x, then y, then z of the box for
+// one population, so y- and z-face transfers are contiguous runs.
 struct CopyOp {
     const void *src;
     void *dst;
-    long long src_plane, dst_plane;
+    long long src_plane, dst_plane;  // in elements
     int src_nx, src_ny, dst_nx, dst_ny;
     int so[3], dof[3], sz[3];
     uint32_t mask;
+    int npop;
+    unsigned char pops[20];
     int block_begin;
     int cells;
 };
 
-template <typename E>
-__global__ void exchange_kernel(const CopyOp *__restrict__ ops, int nops) {
+template <unsigned B>
+__global__ void __launch_bounds__(256) exchange_kernel(const CopyOp *__restrict__ ops, int nops) {
     int k = 0;
     while (k + 1 < nops && ops[k + 1].block_begin <= (int)blockIdx.x) ++k;
     const CopyOp &op = ops[k];
-    const int e = ((int)blockIdx.x - op.block_begin) * (int)blockDim.x + (int)threadIdx.x;
-    if (e >= op.cells) return;
+    const int t = ((int)blockIdx.x - op.block_begin) * (int)blockDim.x + (int)threadIdx.x;
+    if (t >= op.cells * op.npop) return;
+    const int p = t / op.cells;
+    const int e = t - p * op.cells;
     const int bx = e % op.sz[0];
-    const int by = (e / op.sz[0]) % op.sz[1];
-    const int bz = e / (op.sz[0] * op.sz[1]);
-    const long long gs = (long long)(op.so[0] + bx) + (long long)op.src_nx * ((op.so[1] + by) + (long long)op.src_ny * (op.so[2] + bz));
-    const long long gd = (long long)(op.dof[0] + bx) + (long long)op.dst_nx * ((op.dof[1] + by) + (long long)op.dst_ny * (op.dof[2] + bz));
-    const E *s = (const E *)op.src;
-    E *d = (E *)op.dst;
-#pragma unroll
-    for (int i = 0; i < 19; ++i)
-        if (op.mask & (1u << i)) d[(size_t)i * op.dst_plane + gd] = s[(size_t)i * op.src_plane + gs];
+    const int r = e / op.sz[0];
+    const int by = r % op.sz[1];
+    const int bz = r / op.sz[1];
+    const unsigned gs = (unsigned)((op.so[0] + bx) + op.src_nx * ((op.so[1] + by) + op.src_ny * (op.so[2] + bz)));
+    const unsigned gd = (unsigned)((op.dof[0] + bx) + op.dst_nx * ((op.dof[1] + by) + op.dst_ny * (op.dof[2] + bz)));
+    const int i = op.pops[p];
+    const gchar *sp = (const gchar *)op.src + (size_t)i * (size_t)op.src_plane * B;
+    gchar *dp = (gchar *)op.dst + (size_t)i * (size_t)op.dst_plane * B;
+    if (B == 8)
+        *(__attribute__((address_space(1))) unsigned long long *)(dp + gd * B) =
+            *(const __attribute__((address_space(1))) unsigned long long *)(sp + gs * B);
+    else if (B == 4)
+        *(__attribute__((address_space(1))) unsigned *)(dp + gd * B) =
+            *(const __attribute__((address_space(1))) unsigned *)(sp + gs * B);
+    else
+        *(gushort *)(dp + gd * B) = *(const gushort *)(sp + gs * B);
 }
 
 // rho and j = sum f_i e_i read in the current step parity: after an even
@@ -196,13 +209,11 @@ template <int ST>
 __device__ __forceinline__ void cell_moments(const DomDev &D, long long g, bool swapped, double &r, double &a,
                                              double &bb, double &c) {
     using SP = Storage<ST>;
-    using E = typename SP::E;
-    const E *dd = (const E *)D.dd;
     r = a = bb = c = 0.0;
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const int slot = swapped ? opp(i) : i;
-        const double v = (double)SP::load(dd + (size_t)slot * D.plane + g, slot);
+        const double v = (double)SP::ld(plane_ptr<SP>(D, slot), (unsigned)g * SP::kBytes, slot);
         r += v;
         a += v * ex(i);
         bb += v * ey(i);
@@ -362,8 +373,8 @@ struct lbm_group {
         int kind;
     };
     std::vector<Ev> events;
-    double acc_ms[2] = {0, 0};
-    long long acc_n[2] = {0, 0};
+    double acc_ms[3] = {0, 0, 0};   // beta kernel, exchange, alpha kernel
+    long long acc_n[3] = {0, 0, 0};
 };
 
 namespace {
@@ -510,6 +521,10 @@ extern "C" int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out)
         if (c.bc[i] < 0 || c.bc[i] > 255)
             return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "boundary flags must be in [0, 255]");
     if (c.storage < 0 || c.storage > 3) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "unknown storage");
+    // kernels address a plane with 32-bit byte offsets
+    if ((unsigned long long)cells * elem_size(c.storage) >= (1ull << 32))
+        return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
+                      "one distribution plane must stay below 4 GiB: split the domain into subdomains");
     if (c.forcing != 0 && c.forcing != 1) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "unknown forcing");
     int wg = c.work_group_size == 0 ? 256 : c.work_group_size;
     if (wg < 64 || wg > 1024 || wg % 64 != 0)
@@ -843,21 +858,24 @@ static int fill_op(lbm_solver *src, const int *so, lbm_solver *dst, const int *d
         op.sz[a] = size[a];
     }
     op.mask = mask;
-    op.cells = size[0] * size[1] * size[2];
+    op.npop = 0;
+    for (int i = 0; i < 19; ++i)
+        if (mask & (1u << i)) op.pops[op.npop++] = (unsigned char)i;
+    const long long cells = (long long)size[0] * size[1] * size[2];
+    if (cells * op.npop > (1ll << 30)) {
+        why = "ghost-layer box too large for one transfer";
+        return LBM_ERR_OUT_OF_RANGE;
+    }
+    op.cells = (int)cells;
     op.block_begin = 0;
     return LBM_OK;
 }
 
-template <typename E>
-static void launch_exchange(const CopyOp *d_ops, int nops, int nblocks, hipStream_t stream) {
-    exchange_kernel<E><<<nblocks, 256, 0, stream>>>(d_ops, nops);
-}
-
 static hipError_t exchange_launch(int storage, const CopyOp *d_ops, int nops, int nblocks, hipStream_t stream) {
     switch (elem_size(storage)) {
-        case 8: launch_exchange<uint64_t>(d_ops, nops, nblocks, stream); break;
-        case 4: launch_exchange<uint32_t>(d_ops, nops, nblocks, stream); break;
-        default: launch_exchange<uint16_t>(d_ops, nops, nblocks, stream); break;
+        case 8: exchange_kernel<8><<<nblocks, 256, 0, stream>>>(d_ops, nops); break;
+        case 4: exchange_kernel<4><<<nblocks, 256, 0, stream>>>(d_ops, nops); break;
+        default: exchange_kernel<2><<<nblocks, 256, 0, stream>>>(d_ops, nops); break;
     }
     return hipGetLastError();
 }
@@ -872,13 +890,57 @@ extern "C" int lbm_copy_box(lbm_solver *src, const int src_origin[3], lbm_solver
     std::string why;
     int rc = fill_op(src, src_origin, dst, dst_origin, size, norm_mask(norm), op, why);
     if (rc) return fail(s, rc, why);
+    if (op.npop == 0) return LBM_OK;
     SCHK(hipStreamSynchronize(src->stream));
     CopyOp *d_op = nullptr;
     SCHK(hipMalloc((void **)&d_op, sizeof(CopyOp)));
     SCHK(hipMemcpyAsync(d_op, &op, sizeof(CopyOp), hipMemcpyHostToDevice, s->stream));
-    SCHK(exchange_launch(s->cfg.storage, d_op, 1, (op.cells + 255) / 256, s->stream));
+    SCHK(exchange_launch(s->cfg.storage, d_op, 1, (op.cells * op.npop + 255) / 256, s->stream));
     SCHK(hipStreamSynchronize(s->stream));
     SCHK(hipFree(d_op));
+    return LBM_OK;
+}
+
+// ---- device copy bandwidth (calibration of the roofline) --------------------
+
+__global__ void __launch_bounds__(256) stream_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                          long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int lbm_device_copy_bandwidth(int device, size_t bytes, int reps, double *gbs, double *kernel_ms) {
+    if (bytes < 4096 || reps <= 0 || !gbs) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad probe arguments");
+    bytes &= ~(size_t)15;
+    hipError_t e = hipSetDevice(device);
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    const long long n = (long long)(bytes / 16);
+    const int blocks = 256 * 32;
+    if (e == hipSuccess) e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        stream_copy_kernel<<<blocks, 256>>>((const float4 *)a, (float4 *)b, n);
+        e = hipEventRecord(e0, 0);
+    }
+    if (e == hipSuccess) {
+        for (int r = 0; r < reps; ++r) stream_copy_kernel<<<blocks, 256>>>((const float4 *)a, (float4 *)b, n);
+        e = hipEventRecord(e1, 0);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return record(nullptr, LBM_ERR_DEVICE, std::string("copy probe: ") + hipGetErrorString(e));
+    const double per = (double)ms / reps;
+    *gbs = 2.0 * (double)(n * 16) / (per * 1e-3) / 1e9;
+    if (kernel_ms) *kernel_ms = per;
     return LBM_OK;
 }
 
@@ -1056,19 +1118,19 @@ static int build_plan(lbm_group *g) {
         CopyOp a;
         int rc = fill_op(nb, p->send_origin, me, c.recv_origin, c.size, amask, a, why);
         if (rc) return gfail(g, rc, why);
-        ops[0][axis].push_back(a);
+        if (a.npop) ops[0][axis].push_back(a);
         // after beta: neighbour's ghost layer -> my inner layer, n.e > 0 only (src/CController.hpp:322-383)
         CopyOp b;
         rc = fill_op(nb, p->recv_origin, me, c.send_origin, c.size, norm_mask(c.normal), b, why);
         if (rc) return gfail(g, rc, why);
-        ops[1][axis].push_back(b);
+        if (b.npop) ops[1][axis].push_back(b);
     }
     for (int k = 0; k < 2; ++k)
         for (int p = 0; p < 3; ++p) {
             int blocks = 0;
             for (auto &op : ops[k][p]) {
                 op.block_begin = blocks;
-                blocks += (op.cells + 255) / 256;
+                blocks += (op.cells * op.npop + 255) / 256;
             }
             g->nops[k][p] = (int)ops[k][p].size();
             g->nblocks[k][p] = blocks;
@@ -1099,7 +1161,7 @@ static int group_launch_half(lbm_group *g, bool beta, bool timed) {
     if (rc) return rc;
     if (timed) {
         GCHK(hipEventRecord(eb, g->stream));
-        g->events.push_back({ea, eb, 0});
+        g->events.push_back({ea, eb, beta ? 0 : 2});
     }
     // the kernel just run was beta -> syncBeta, alpha -> syncAlpha (src/CController.hpp:385-391)
     const int k = beta ? 1 : 0;
@@ -1195,14 +1257,15 @@ extern "C" int lbm_group_set_timing(lbm_group *g, int enable) {
         (void)hipEventDestroy(e.b);
     }
     g->events.clear();
-    g->acc_ms[0] = g->acc_ms[1] = 0;
-    g->acc_n[0] = g->acc_n[1] = 0;
+    for (int k = 0; k < 3; ++k) {
+        g->acc_ms[k] = 0;
+        g->acc_n[k] = 0;
+    }
     g->timing = enable != 0;
     return LBM_OK;
 }
 
-extern "C" int lbm_group_get_timing(lbm_group *g, double *step_ms, long long *step_n, double *xch_ms, long long *xch_n) {
-    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+static int drain_events(lbm_group *g) {
     GCHK(hipStreamSynchronize(g->stream));
     for (auto &e : g->events) {
         float ms = 0.f;
@@ -1213,8 +1276,26 @@ extern "C" int lbm_group_get_timing(lbm_group *g, double *step_ms, long long *st
         (void)hipEventDestroy(e.b);
     }
     g->events.clear();
-    if (step_ms) *step_ms = g->acc_n[0] ? g->acc_ms[0] / g->acc_n[0] : 0.0;
-    if (step_n) *step_n = g->acc_n[0];
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_get_timing_detail(lbm_group *g, double *beta_ms, double *alpha_ms, double *exchange_ms) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    int rc = drain_events(g);
+    if (rc) return rc;
+    if (beta_ms) *beta_ms = g->acc_n[0] ? g->acc_ms[0] / g->acc_n[0] : 0.0;
+    if (alpha_ms) *alpha_ms = g->acc_n[2] ? g->acc_ms[2] / g->acc_n[2] : 0.0;
+    if (exchange_ms) *exchange_ms = g->acc_n[1] ? g->acc_ms[1] / g->acc_n[1] : 0.0;
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_get_timing(lbm_group *g, double *step_ms, long long *step_n, double *xch_ms, long long *xch_n) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    int rc = drain_events(g);
+    if (rc) return rc;
+    const long long ns = g->acc_n[0] + g->acc_n[2];
+    if (step_ms) *step_ms = ns ? (g->acc_ms[0] + g->acc_ms[2]) / ns : 0.0;
+    if (step_n) *step_n = ns;
     if (xch_ms) *xch_ms = g->acc_n[1] ? g->acc_ms[1] / g->acc_n[1] : 0.0;
     if (xch_n) *xch_n = g->acc_n[1];
     return LBM_OK;

@@ -250,8 +250,12 @@ class LbmGroup:
         a, b = C.c_double(), C.c_double()
         na, nb = C.c_longlong(), C.c_longlong()
         self._check(self._lib.lbm_group_get_timing(self._h, C.byref(a), C.byref(na), C.byref(b), C.byref(nb)))
-        return {"step_kernel_ms": a.value, "step_kernel_launches": na.value,
-                "exchange_ms": b.value, "exchange_launches": nb.value}
+        out = {"step_kernel_ms": a.value, "step_kernel_launches": na.value,
+               "exchange_ms": b.value, "exchange_launches": nb.value}
+        be, al, xc = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._lib.lbm_group_get_timing_detail(self._h, C.byref(be), C.byref(al), C.byref(xc)))
+        out.update(beta_kernel_ms=be.value, alpha_kernel_ms=al.value)
+        return out
 
     def close(self):
         if getattr(self, "_h", None):
