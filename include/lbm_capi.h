@@ -83,6 +83,8 @@ typedef struct lbm_solver_config {
     int work_group_size;    /* threads per work-group of the step kernels; 0 = 256 */
     int plane_padding;      /* extra elements per distribution plane; -1 = default */
     int device;             /* HIP device ordinal */
+    int kernel_variant;     /* tuning bits: 1 non-temporal loads, 2 non-temporal stores, 4 no XCD tile map */
+    int lds_per_block;      /* bytes of LDS reserved per work-group (caps residency); 0 = none */
 } lbm_solver_config;
 
 /* ---- library ---------------------------------------------------------- */
@@ -91,11 +93,12 @@ const char *lbm_version(void);
 const char *lbm_last_error(void);
 int lbm_device_synchronize(int device);
 void lbm_solver_config_default(lbm_solver_config *cfg);
-/* read+write bandwidth of a plain device copy of `bytes` (stream_copy_kernel),
- * averaged over `reps` launches: the achievable-HBM reference of the roofline
- * and the known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters
- * (new; no reference counterpart) */
-int lbm_device_copy_bandwidth(int device, size_t bytes, int reps, double *gbs, double *kernel_ms);
+/* read+write bandwidth of a plain device copy of `bytes`, averaged over `reps`
+ * launches (variant 0: grid-stride float4, 1: four float4 per work-item,
+ * 2: as 1 with non-temporal loads/stores): the achievable-HBM reference of the
+ * roofline and the known-size kernel that calibrates the FETCH_SIZE and
+ * WRITE_SIZE counters (new; no reference counterpart) */
+int lbm_device_copy_bandwidth(int device, size_t bytes, int reps, int variant, double *gbs, double *kernel_ms);
 
 /* ---- solver: CLbmSolver<T> (src/CLbmSolver.hpp:67-1124) ----------------- */
 /* constructor + reload() + reset() (src/CLbmSolver.hpp:221-260, 272-635) */

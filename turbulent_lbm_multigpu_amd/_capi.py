@@ -80,6 +80,8 @@ class SolverConfig(C.Structure):
         ("work_group_size", C.c_int),
         ("plane_padding", C.c_int),
         ("device", C.c_int),
+        ("kernel_variant", C.c_int),
+        ("lds_per_block", C.c_int),
     ]
 
 
@@ -112,7 +114,7 @@ def load_library(path: str | None = None):
     sig("lbm_last_error", C.c_char_p)
     sig("lbm_device_synchronize", I, I)
     sig("lbm_solver_config_default", None, C.POINTER(SolverConfig))
-    sig("lbm_device_copy_bandwidth", I, I, C.c_size_t, I, C.POINTER(D), C.POINTER(D))
+    sig("lbm_device_copy_bandwidth", I, I, C.c_size_t, I, I, C.POINTER(D), C.POINTER(D))
     sig("lbm_solver_create", I, C.POINTER(SolverConfig), C.POINTER(P))
     sig("lbm_solver_destroy", I, P)
     sig("lbm_solver_error", C.c_char_p, P)

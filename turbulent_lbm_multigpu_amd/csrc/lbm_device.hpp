@@ -82,6 +82,20 @@ __device__ __forceinline__ unsigned short bf16_bits_rne(float v) {
     return (unsigned short)(u >> 16);
 }
 
+// Raw element access; NT selects the non-temporal (streaming) cache policy.
+template <bool NT, typename U>
+__device__ __forceinline__ U raw_ld(const gchar *base, unsigned off) {
+    const __attribute__((address_space(1))) U *p = (const __attribute__((address_space(1))) U *)(base + off);
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+template <bool NT, typename U>
+__device__ __forceinline__ void raw_st(gchar *base, unsigned off, U v) {
+    __attribute__((address_space(1))) U *p = (__attribute__((address_space(1))) U *)(base + off);
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 template <int ST>
 struct Storage;
 template <>
@@ -89,27 +103,33 @@ struct Storage<0> {
     using E = float;
     using T = float;
     static constexpr unsigned kBytes = 4;
-    __device__ static T ld(const gchar *base, unsigned off, int) { return *(const gfloat *)(base + off); }
-    __device__ static void st(gchar *base, unsigned off, int, T v) { *(gfloat *)(base + off) = v; }
+    template <bool NT = false>
+    __device__ static T ld(const gchar *base, unsigned off, int) { return raw_ld<NT, float>(base, off); }
+    template <bool NT = false>
+    __device__ static void st(gchar *base, unsigned off, int, T v) { raw_st<NT, float>(base, off, v); }
 };
 template <>
 struct Storage<1> {
     using E = double;
     using T = double;
     static constexpr unsigned kBytes = 8;
-    __device__ static T ld(const gchar *base, unsigned off, int) { return *(const gdouble *)(base + off); }
-    __device__ static void st(gchar *base, unsigned off, int, T v) { *(gdouble *)(base + off) = v; }
+    template <bool NT = false>
+    __device__ static T ld(const gchar *base, unsigned off, int) { return raw_ld<NT, double>(base, off); }
+    template <bool NT = false>
+    __device__ static void st(gchar *base, unsigned off, int, T v) { raw_st<NT, double>(base, off, v); }
 };
 template <>
 struct Storage<2> {
     using E = __half;
     using T = float;
     static constexpr unsigned kBytes = 2;
+    template <bool NT = false>
     __device__ static T ld(const gchar *base, unsigned off, int i) {
-        return __half2float(__ushort_as_half(*(const gushort *)(base + off))) + weight<float>(i);
+        return __half2float(__ushort_as_half(raw_ld<NT, unsigned short>(base, off))) + weight<float>(i);
     }
+    template <bool NT = false>
     __device__ static void st(gchar *base, unsigned off, int i, T v) {
-        *(gushort *)(base + off) = __half_as_ushort(__float2half(v - weight<float>(i)));
+        raw_st<NT, unsigned short>(base, off, __half_as_ushort(__float2half(v - weight<float>(i))));
     }
 };
 template <>
@@ -117,13 +137,20 @@ struct Storage<3> {
     using E = __hip_bfloat16;
     using T = float;
     static constexpr unsigned kBytes = 2;
+    template <bool NT = false>
     __device__ static T ld(const gchar *base, unsigned off, int i) {
-        return __uint_as_float((unsigned)(*(const gushort *)(base + off)) << 16) + weight<float>(i);
+        return __uint_as_float((unsigned)raw_ld<NT, unsigned short>(base, off) << 16) + weight<float>(i);
     }
+    template <bool NT = false>
     __device__ static void st(gchar *base, unsigned off, int i, T v) {
-        *(gushort *)(base + off) = bf16_bits_rne(v - weight<float>(i));
+        raw_st<NT, unsigned short>(base, off, bf16_bits_rne(v - weight<float>(i)));
     }
 };
+
+// Step-kernel variants (tuning knobs): bit 0 non-temporal loads, bit 1
+// non-temporal stores.
+constexpr int kVarNtLoad = 1;
+constexpr int kVarNtStore = 2;
 
 // ---- per-(sub)domain launch descriptor ---------------------------------------
 struct DomDev {
@@ -304,33 +331,35 @@ __device__ __forceinline__ int wrap_idx(int a, int cells) {
 // blocks whose cells are all at least max|d| away from both ends of the
 // domain, where the linear wrap W (src/cl_programs/wrap.h:112-127) is the
 // identity.
-template <class SP, bool WRAP>
+template <class SP, bool WRAP, bool NT>
 __device__ __forceinline__ void beta_gather(gchar *const *pl, unsigned g, int cells, const int *d,
                                             typename SP::T *f) {
 #pragma unroll
     for (int i = 0; i < 18; ++i) {
         unsigned idx = WRAP ? (unsigned)wrap_idx((int)g - d[i], cells) : g - (unsigned)d[i];
-        f[i] = SP::ld(pl[opp(i)], idx * SP::kBytes, opp(i));
+        f[i] = SP::template ld<NT>(pl[opp(i)], idx * SP::kBytes, opp(i));
     }
-    f[18] = SP::ld(pl[18], g * SP::kBytes, 18);
+    f[18] = SP::template ld<NT>(pl[18], g * SP::kBytes, 18);
 }
 
-template <class SP, bool WRAP>
+template <class SP, bool WRAP, bool NT>
 __device__ __forceinline__ void beta_scatter(gchar *const *pl, unsigned g, int cells, const int *d,
                                              const typename SP::T *f) {
 #pragma unroll
     for (int i = 0; i < 18; ++i) {
         unsigned idx = WRAP ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
-        SP::st(pl[i], idx * SP::kBytes, i, f[i]);
+        SP::template st<NT>(pl[i], idx * SP::kBytes, i, f[i]);
     }
-    SP::st(pl[18], g * SP::kBytes, 18, f[18]);
+    SP::template st<NT>(pl[18], g * SP::kBytes, 18, f[18]);
 }
 
-template <int ST, bool BETA>
+template <int ST, bool BETA, int VAR = 0>
 __global__ void __launch_bounds__(1024) step_kernel(const DomDev *__restrict__ doms, const int *__restrict__ tile_dom,
                                                     int xcd_map) {
     using SP = Storage<ST>;
     using T = typename SP::T;
+    constexpr bool NTL = (VAR & kVarNtLoad) != 0;
+    constexpr bool NTS = (VAR & kVarNtStore) != 0;
     const int nb = (int)gridDim.x;
     const int t = xcd_map ? xcd_tile((int)blockIdx.x, nb) : (int)blockIdx.x;
     const DomDev *D = doms + tile_dom[t];
@@ -358,14 +387,14 @@ __global__ void __launch_bounds__(1024) step_kernel(const DomDev *__restrict__ d
         const int dmax = 1 + X + XY;
         wrap = block_first < dmax || block_first + (int)blockDim.x + dmax > cells;
         if (wrap)
-            beta_gather<SP, true>(pl, g, cells, d, f);
+            beta_gather<SP, true, NTL>(pl, g, cells, d, f);
         else
-            beta_gather<SP, false>(pl, g, cells, d, f);
+            beta_gather<SP, false, NTL>(pl, g, cells, d, f);
     } else {
         // the 19 loads do not wait for the flag; a GHOST cell (lbm_alpha.cl:30-32)
         // matches no branch below and stores nothing
 #pragma unroll
-        for (int i = 0; i < 19; ++i) f[i] = SP::ld(pl[i], g * SP::kBytes, i);
+        for (int i = 0; i < 19; ++i) f[i] = SP::template ld<NTL>(pl[i], g * SP::kBytes, i);
     }
 
     T rho, vx, vy, vz;
@@ -448,12 +477,12 @@ __global__ void __launch_bounds__(1024) step_kernel(const DomDev *__restrict__ d
     if (write_dd) {
         if (BETA) {
             if (wrap)
-                beta_scatter<SP, true>(pl, g, cells, d, f);
+                beta_scatter<SP, true, NTS>(pl, g, cells, d, f);
             else
-                beta_scatter<SP, false>(pl, g, cells, d, f);
+                beta_scatter<SP, false, NTS>(pl, g, cells, d, f);
         } else {
 #pragma unroll
-            for (int i = 0; i < 19; ++i) SP::st(pl[opp(i)], g * SP::kBytes, opp(i), f[i]);
+            for (int i = 0; i < 19; ++i) SP::template st<NTS>(pl[opp(i)], g * SP::kBytes, opp(i), f[i]);
         }
     }
     if (flag == kFlagGhost) return;  // lbm_alpha.cl:30-32, lbm_beta.cl:787-788

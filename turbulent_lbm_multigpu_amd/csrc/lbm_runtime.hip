@@ -91,7 +91,7 @@ __global__ void init_kernel(DomDev D, InitArgs a) {
     T f[19];
     feq19(dd_param, u[0], u[1], u[2], f);
 #pragma unroll
-    for (int i = 0; i < 19; ++i) SP::st(plane_ptr<SP>(D, opp(i)), (unsigned)g * SP::kBytes, opp(i), f[i]);
+    for (int i = 0; i < 19; ++i) SP::template st<false>(plane_ptr<SP>(D, opp(i)), (unsigned)g * SP::kBytes, opp(i), f[i]);
     if (D.vel) {
         T *v = (T *)D.vel;
         v[g] = u[0];
@@ -121,7 +121,7 @@ __global__ void pack_kernel(DomDev D, int field, Box b, void *out) {
     if (field == LBM_FIELD_DD) {
         T *o = (T *)out;
 #pragma unroll
-        for (int i = 0; i < 19; ++i) o[i * n + e] = SP::ld(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i);
+        for (int i = 0; i < 19; ++i) o[i * n + e] = SP::template ld<false>(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i);
     } else if (field == LBM_FIELD_VELOCITY) {
         const T *v = (const T *)D.vel;
         T *o = (T *)out;
@@ -146,7 +146,7 @@ __global__ void unpack_kernel(DomDev D, int field, Box b, const void *in, uint32
         const T *s = (const T *)in;
 #pragma unroll
         for (int i = 0; i < 19; ++i)
-            if (mask & (1u << i)) SP::st(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i, s[i * n + e]);
+            if (mask & (1u << i)) SP::template st<false>(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i, s[i * n + e]);
     } else if (field == LBM_FIELD_VELOCITY) {
         T *v = (T *)D.vel;
         const T *s = (const T *)in;
@@ -213,7 +213,7 @@ __device__ __forceinline__ void cell_moments(const DomDev &D, long long g, bool 
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const int slot = swapped ? opp(i) : i;
-        const double v = (double)SP::ld(plane_ptr<SP>(D, slot), (unsigned)g * SP::kBytes, slot);
+        const double v = (double)SP::template ld<false>(plane_ptr<SP>(D, slot), (unsigned)g * SP::kBytes, slot);
         r += v;
         a += v * ex(i);
         bb += v * ey(i);
@@ -437,17 +437,39 @@ int upload_desc(lbm_solver *s) {
     return LBM_OK;
 }
 
-int launch_step(lbm_solver *s, bool beta) {
-    const int storage = s->cfg.storage;
-    return dispatch_storage(storage, [&](auto st) -> int {
+// Launch of the step kernel: storage type, half-step and variant bits become
+// template arguments; bit 2 of the variant turns the XCD-contiguous tile map
+// off; lds_per_block reserves LDS to cap the number of resident work-groups.
+hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
+                              const int *tile_table, hipStream_t stream) {
+    const int xcd = (variant & 4) ? 0 : 1;
+    const int v = variant & 3;
+    dispatch_storage(storage, [&](auto st) -> int {
         constexpr int ST = decltype(st)::value;
-        if (beta)
-            step_kernel<ST, true><<<s->ntiles, s->wg, 0, s->stream>>>(s->d_desc, s->d_tile, 1);
-        else
-            step_kernel<ST, false><<<s->ntiles, s->wg, 0, s->stream>>>(s->d_desc, s->d_tile, 1);
-        SCHK(hipGetLastError());
-        return LBM_OK;
+        auto go = [&](auto b, auto vv) {
+            constexpr bool B = decltype(b)::value;
+            constexpr int V = decltype(vv)::value;
+            step_kernel<ST, B, V><<<tiles, wg, lds, stream>>>(descs, tile_table, xcd);
+        };
+        auto pick = [&](auto b) {
+            switch (v) {
+                case 0: go(b, std::integral_constant<int, 0>()); break;
+                case 1: go(b, std::integral_constant<int, 1>()); break;
+                case 2: go(b, std::integral_constant<int, 2>()); break;
+                default: go(b, std::integral_constant<int, 3>()); break;
+            }
+        };
+        if (beta) pick(std::true_type());
+        else pick(std::false_type());
+        return 0;
     });
+    return hipGetLastError();
+}
+
+int launch_step(lbm_solver *s, bool beta) {
+    SCHK(launch_step_kernel(s->cfg.storage, beta, s->cfg.kernel_variant, s->cfg.lds_per_block, s->ntiles, s->wg,
+                            s->d_desc, s->d_tile, s->stream));
+    return LBM_OK;
 }
 
 int ensure_staging(lbm_solver *s, size_t bytes) {
@@ -533,6 +555,8 @@ extern "C" int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out)
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "inv_tau must be positive and finite");
     if (c.smagorinsky_cs < 0.0 || (c.smagorinsky_cs > 0.0 && !(c.tau > 0.5)))
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "LES needs smagorinsky_cs >= 0 and tau > 0.5");
+    if (c.kernel_variant < 0 || c.kernel_variant > 7 || c.lds_per_block < 0 || c.lds_per_block > 65536)
+        return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "kernel_variant must be in [0, 7], lds_per_block in [0, 65536]");
 
     lbm_solver *s = new lbm_solver();
     s->cfg = c;
@@ -909,26 +933,55 @@ __global__ void __launch_bounds__(256) stream_copy_kernel(const float4 *__restri
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
-extern "C" int lbm_device_copy_bandwidth(int device, size_t bytes, int reps, double *gbs, double *kernel_ms) {
-    if (bytes < 4096 || reps <= 0 || !gbs) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad probe arguments");
+// Each work-item moves 4 float4 spaced one block apart: four loads in flight,
+// then four stores; NT selects the non-temporal cache policy.
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <bool NT>
+__global__ void __launch_bounds__(256) stream_copy4_kernel(const v4f *__restrict__ src, v4f *__restrict__ dst,
+                                                           long long n) {
+    const long long base = (long long)blockIdx.x * blockDim.x * 4 + threadIdx.x;
+    v4f v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long i = base + (long long)k * blockDim.x;
+        if (i < n) v[k] = NT ? __builtin_nontemporal_load(src + i) : src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long i = base + (long long)k * blockDim.x;
+        if (i < n) {
+            if (NT) __builtin_nontemporal_store(v[k], dst + i);
+            else dst[i] = v[k];
+        }
+    }
+}
+
+extern "C" int lbm_device_copy_bandwidth(int device, size_t bytes, int reps, int variant, double *gbs,
+                                         double *kernel_ms) {
+    if (bytes < 4096 || reps <= 0 || !gbs || variant < 0 || variant > 2)
+        return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad probe arguments");
     bytes &= ~(size_t)15;
     hipError_t e = hipSetDevice(device);
     void *a = nullptr, *b = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float ms = 0.f;
     const long long n = (long long)(bytes / 16);
-    const int blocks = 256 * 32;
+    auto launch = [&]() {
+        if (variant == 0) stream_copy_kernel<<<256 * 32, 256>>>((const float4 *)a, (float4 *)b, n);
+        else if (variant == 1) stream_copy4_kernel<false><<<(unsigned)((n + 1023) / 1024), 256>>>((const v4f *)a, (v4f *)b, n);
+        else stream_copy4_kernel<true><<<(unsigned)((n + 1023) / 1024), 256>>>((const v4f *)a, (v4f *)b, n);
+    };
     if (e == hipSuccess) e = hipMalloc(&a, bytes);
     if (e == hipSuccess) e = hipMalloc(&b, bytes);
     if (e == hipSuccess) e = hipMemset(a, 0, bytes);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) {
-        stream_copy_kernel<<<blocks, 256>>>((const float4 *)a, (float4 *)b, n);
+        launch();
         e = hipEventRecord(e0, 0);
     }
     if (e == hipSuccess) {
-        for (int r = 0; r < reps; ++r) stream_copy_kernel<<<blocks, 256>>>((const float4 *)a, (float4 *)b, n);
+        for (int r = 0; r < reps; ++r) launch();
         e = hipEventRecord(e1, 0);
     }
     if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -1149,16 +1202,9 @@ static int group_launch_half(lbm_group *g, bool beta, bool timed) {
         GCHK(hipEventCreate(&eb));
         GCHK(hipEventRecord(ea, g->stream));
     }
-    int rc = dispatch_storage(g->storage, [&](auto st) -> int {
-        constexpr int ST = decltype(st)::value;
-        if (beta)
-            step_kernel<ST, true><<<g->ntiles, g->wg, 0, g->stream>>>(g->d_descs, g->d_tiles, 1);
-        else
-            step_kernel<ST, false><<<g->ntiles, g->wg, 0, g->stream>>>(g->d_descs, g->d_tiles, 1);
-        GCHK(hipGetLastError());
-        return LBM_OK;
-    });
-    if (rc) return rc;
+    const lbm_solver_config &c0 = g->members[0]->cfg;
+    GCHK(launch_step_kernel(g->storage, beta, c0.kernel_variant, c0.lds_per_block, g->ntiles, g->wg, g->d_descs,
+                            g->d_tiles, g->stream));
     if (timed) {
         GCHK(hipEventRecord(eb, g->stream));
         g->events.push_back({ea, eb, beta ? 0 : 2});
