@@ -80,7 +80,7 @@ typedef struct lbm_solver_config {
     double lid_velocity;    /* lattice lid speed (x) of VELOCITY_INJECTION cells */
     int store_velocity;     /* kernels write the velocity field (STORE_VELOCITY) */
     int store_density;      /* kernels write the density field (STORE_DENSITY) */
-    int work_group_size;    /* threads per work-group of the step kernels; 0 = 256 */
+    int work_group_size;    /* threads per work-group of the step kernels: 64..256; 0 = 256 */
     int plane_padding;      /* extra elements per distribution plane; -1 = default */
     int device;             /* HIP device ordinal */
     int kernel_variant;     /* tuning bits: 1 non-temporal loads, 2 non-temporal stores, 4 no XCD tile map */

@@ -36,13 +36,13 @@ def rel_err(a, b):
 
 
 def make_pair(dims, bc=(1,) * 6, *, inv_tau, tau=0.0, cs=0.0, lid=0.0, g=(0.0, 0.0, 0.0), forcing=0,
-              storage=_capi.STORAGE_F32, wg=256, lid_layer=True, amp=0.0, seed=0, store=True):
+              storage=_capi.STORAGE_F32, wg=256, lid_layer=True, amp=0.0, seed=0, store=True, variant=0):
     dtype = np.float64 if storage == _capi.STORAGE_F64 else np.float32
     o = orc.OracleDomain(dims, bc, inv_tau=inv_tau, tau=tau, smagorinsky_cs=cs, lid_velocity=lid,
                          gravitation=g, forcing=forcing, dtype=dtype)
     s = LbmSolver(dims, bc, inv_tau=inv_tau, tau=tau, smagorinsky_cs=cs, lid_velocity=lid, gravitation=g,
                   forcing=forcing, storage=storage, work_group_size=wg, store_velocity=store,
-                  store_density=store)
+                  store_density=store, kernel_variant=variant)
     if amp:
         o.reset(amplitude=amp, seed=seed)
         s.init_perturbed(amp, seed)
@@ -61,11 +61,14 @@ def test_init_matches_oracle():
     s.close()
 
 
+VARIANTS = [0, 2, 4, 8, 16, 16 | 4]
+
+
 @pytest.mark.parametrize("steps", [1, 2, 3, 8])
-@pytest.mark.parametrize("wg", [64, 256, 1024])
-def test_bgk_cavity_bitwise(steps, wg):
-    say(f"case: bgk cavity 19x14x12 steps={steps} wg={wg}")
-    o, s = make_pair((19, 14, 12), inv_tau=1 / 0.6, lid=0.05, wg=wg, amp=1e-3, seed=3)
+@pytest.mark.parametrize("wg,variant", [(64, 0), (256, 0), (128, 8), (256, 16), (64, 16)])
+def test_bgk_cavity_bitwise(steps, wg, variant):
+    say(f"case: bgk cavity 19x14x12 steps={steps} wg={wg} variant={variant}")
+    o, s = make_pair((19, 14, 12), inv_tau=1 / 0.6, lid=0.05, wg=wg, amp=1e-3, seed=3, variant=variant)
     o.run(steps)
     s.run(steps)
     np.testing.assert_array_equal(gpu_dd(s), o.dd)
@@ -75,18 +78,22 @@ def test_bgk_cavity_bitwise(steps, wg):
 
 
 @pytest.mark.parametrize("steps", [2, 5])
-def test_les_bitwise(steps):
-    say(f"case: LES 24x20x16 steps={steps}")
-    o, s = make_pair((24, 20, 16), inv_tau=1 / 0.5005, tau=0.5005, cs=0.17, lid=0.1, amp=1e-2, seed=20260101)
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_les_bitwise(steps, variant):
+    say(f"case: LES 40x36x32 steps={steps} variant={variant}")
+    o, s = make_pair((40, 36, 32), inv_tau=1 / 0.5005, tau=0.5005, cs=0.17, lid=0.1, amp=1e-2, seed=20260101,
+                     variant=variant)
     o.run(steps)
     s.run(steps)
     np.testing.assert_array_equal(gpu_dd(s), o.dd)
     s.close()
 
 
-def test_ghost_faces_and_obstacles_bitwise():
-    say("case: ghost faces + interior obstacle 16x12x10")
-    o, s = make_pair((16, 12, 10), (8, 8, 1, 8, 8, 1), inv_tau=1 / 0.7, lid=0.04, amp=1e-2, seed=8)
+@pytest.mark.parametrize("variant", [0, 8, 16])
+def test_ghost_faces_and_obstacles_bitwise(variant):
+    say(f"case: ghost faces + interior obstacle 36x30x28 variant={variant}")
+    o, s = make_pair((36, 30, 28), (8, 8, 1, 8, 8, 1), inv_tau=1 / 0.7, lid=0.04, amp=1e-2, seed=8,
+                     variant=variant)
     blk = np.full((2, 3, 4), orc.FLAG_OBSTACLE, np.int32)
     o.set_flags((5, 4, 3), (4, 3, 2), orc.FLAG_OBSTACLE)
     s.set(_capi.FIELD_FLAGS, blk, (5, 4, 3), (4, 3, 2))
@@ -98,9 +105,11 @@ def test_ghost_faces_and_obstacles_bitwise():
 
 
 @pytest.mark.parametrize("forcing,g", [(0, (1e-4, -2e-4, 3e-5)), (1, (1e-5, 0.0, -2e-6))])
-def test_forcing_bitwise(forcing, g):
-    say(f"case: forcing={forcing} g={g}")
-    o, s = make_pair((14, 12, 10), inv_tau=1 / 0.8, lid=0.03, g=g, forcing=forcing, amp=1e-3, seed=1)
+@pytest.mark.parametrize("variant", [0, 16])
+def test_forcing_bitwise(forcing, g, variant):
+    say(f"case: forcing={forcing} g={g} variant={variant}")
+    o, s = make_pair((34, 32, 30), inv_tau=1 / 0.8, lid=0.03, g=g, forcing=forcing, amp=1e-3, seed=1,
+                     variant=variant)
     o.run(6)
     s.run(6)
     np.testing.assert_array_equal(gpu_dd(s), o.dd)
@@ -108,10 +117,11 @@ def test_forcing_bitwise(forcing, g):
     s.close()
 
 
-def test_fp64_storage_bitwise():
-    say("case: fp64 LES 16x14x12")
-    o, s = make_pair((16, 14, 12), inv_tau=1 / 0.52, tau=0.52, cs=0.17, lid=0.08, amp=1e-3, seed=4,
-                     storage=_capi.STORAGE_F64)
+@pytest.mark.parametrize("variant", [0, 8])
+def test_fp64_storage_bitwise(variant):
+    say(f"case: fp64 LES 32x30x28 variant={variant}")
+    o, s = make_pair((32, 30, 28), inv_tau=1 / 0.52, tau=0.52, cs=0.17, lid=0.08, amp=1e-3, seed=4,
+                     storage=_capi.STORAGE_F64, variant=variant)
     o.run(9)
     s.run(9)
     np.testing.assert_array_equal(gpu_dd(s), o.dd)

@@ -96,61 +96,76 @@ __device__ __forceinline__ void raw_st(gchar *base, unsigned off, U v) {
     else *p = v;
 }
 
+// Storage traits: U is the stored bit pattern, T the compute type; from_bits /
+// to_bits convert one element of population slot i.
 template <int ST>
 struct Storage;
 template <>
 struct Storage<0> {
-    using E = float;
+    using U = unsigned;
     using T = float;
     static constexpr unsigned kBytes = 4;
-    template <bool NT = false>
-    __device__ static T ld(const gchar *base, unsigned off, int) { return raw_ld<NT, float>(base, off); }
-    template <bool NT = false>
-    __device__ static void st(gchar *base, unsigned off, int, T v) { raw_st<NT, float>(base, off, v); }
+    __device__ static T from_bits(U u, int) { return __uint_as_float(u); }
+    __device__ static U to_bits(T v, int) { return __float_as_uint(v); }
 };
 template <>
 struct Storage<1> {
-    using E = double;
+    using U = unsigned long long;
     using T = double;
     static constexpr unsigned kBytes = 8;
-    template <bool NT = false>
-    __device__ static T ld(const gchar *base, unsigned off, int) { return raw_ld<NT, double>(base, off); }
-    template <bool NT = false>
-    __device__ static void st(gchar *base, unsigned off, int, T v) { raw_st<NT, double>(base, off, v); }
+    __device__ static T from_bits(U u, int) { return __longlong_as_double((long long)u); }
+    __device__ static U to_bits(T v, int) { return (U)__double_as_longlong(v); }
 };
 template <>
 struct Storage<2> {
-    using E = __half;
+    using U = unsigned short;
     using T = float;
     static constexpr unsigned kBytes = 2;
-    template <bool NT = false>
-    __device__ static T ld(const gchar *base, unsigned off, int i) {
-        return __half2float(__ushort_as_half(raw_ld<NT, unsigned short>(base, off))) + weight<float>(i);
-    }
-    template <bool NT = false>
-    __device__ static void st(gchar *base, unsigned off, int i, T v) {
-        raw_st<NT, unsigned short>(base, off, __half_as_ushort(__float2half(v - weight<float>(i))));
-    }
+    __device__ static T from_bits(U u, int i) { return __half2float(__ushort_as_half(u)) + weight<float>(i); }
+    __device__ static U to_bits(T v, int i) { return __half_as_ushort(__float2half(v - weight<float>(i))); }
 };
 template <>
 struct Storage<3> {
-    using E = __hip_bfloat16;
+    using U = unsigned short;
     using T = float;
     static constexpr unsigned kBytes = 2;
-    template <bool NT = false>
-    __device__ static T ld(const gchar *base, unsigned off, int i) {
-        return __uint_as_float((unsigned)raw_ld<NT, unsigned short>(base, off) << 16) + weight<float>(i);
-    }
-    template <bool NT = false>
-    __device__ static void st(gchar *base, unsigned off, int i, T v) {
-        raw_st<NT, unsigned short>(base, off, bf16_bits_rne(v - weight<float>(i)));
-    }
+    __device__ static T from_bits(U u, int i) { return __uint_as_float((unsigned)u << 16) + weight<float>(i); }
+    __device__ static U to_bits(T v, int i) { return bf16_bits_rne(v - weight<float>(i)); }
 };
 
-// Step-kernel variants (tuning knobs): bit 0 non-temporal loads, bit 1
-// non-temporal stores.
-constexpr int kVarNtLoad = 1;
+// one element of population slot i at byte offset `off` of its plane
+template <class SP, bool NT = false>
+__device__ __forceinline__ typename SP::T ld(const gchar *base, unsigned off, int i) {
+    return SP::from_bits(raw_ld<NT, typename SP::U>(base, off), i);
+}
+template <class SP, bool NT = false>
+__device__ __forceinline__ void st(gchar *base, unsigned off, int i, typename SP::T v) {
+    raw_st<NT, typename SP::U>(base, off, SP::to_bits(v, i));
+}
+
+// N consecutive elements as one vector access (alignment: one element).
+template <class SP, int N>
+__device__ __forceinline__ void vld(const gchar *base, unsigned off, int i, typename SP::T *out) {
+    typedef typename SP::U V __attribute__((ext_vector_type(N)));
+    V v;
+    __builtin_memcpy(&v, base + off, sizeof(V));
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = SP::from_bits(v[j], i);
+}
+template <class SP, int N>
+__device__ __forceinline__ void vst(gchar *base, unsigned off, int i, const typename SP::T *in) {
+    typedef typename SP::U V __attribute__((ext_vector_type(N)));
+    V v;
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = SP::to_bits(in[j], i);
+    __builtin_memcpy(base + off, &v, sizeof(V));
+}
+
+// Step-kernel variant bits (tuning knobs): bit 1 non-temporal stores on the
+// one-cell path, bit 2 no XCD tile map (host side), bits 3-4 log2 of the
+// number of x-consecutive cells one work-item updates (1, 2 or 4).
 constexpr int kVarNtStore = 2;
+__host__ __device__ constexpr int cells_per_lane(int var) { return 1 << ((var >> 3) & 3); }
 
 // ---- per-(sub)domain launch descriptor ---------------------------------------
 struct DomDev {
@@ -327,77 +342,13 @@ __device__ __forceinline__ int wrap_idx(int a, int cells) {
 }
 
 // ---- the step kernel -------------------------------------------------------
-// Gather of the beta kernel: f_i = S[opp(i)][W(g - d_i)].  WRAP is false for
-// blocks whose cells are all at least max|d| away from both ends of the
-// domain, where the linear wrap W (src/cl_programs/wrap.h:112-127) is the
-// identity.
-template <class SP, bool WRAP, bool NT>
-__device__ __forceinline__ void beta_gather(gchar *const *pl, unsigned g, int cells, const int *d,
-                                            typename SP::T *f) {
-#pragma unroll
-    for (int i = 0; i < 18; ++i) {
-        unsigned idx = WRAP ? (unsigned)wrap_idx((int)g - d[i], cells) : g - (unsigned)d[i];
-        f[i] = SP::template ld<NT>(pl[opp(i)], idx * SP::kBytes, opp(i));
-    }
-    f[18] = SP::template ld<NT>(pl[18], g * SP::kBytes, 18);
-}
-
-template <class SP, bool WRAP, bool NT>
-__device__ __forceinline__ void beta_scatter(gchar *const *pl, unsigned g, int cells, const int *d,
-                                             const typename SP::T *f) {
-#pragma unroll
-    for (int i = 0; i < 18; ++i) {
-        unsigned idx = WRAP ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
-        SP::template st<NT>(pl[i], idx * SP::kBytes, i, f[i]);
-    }
-    SP::template st<NT>(pl[18], g * SP::kBytes, 18, f[18]);
-}
-
-template <int ST, bool BETA, int VAR = 0>
-__global__ void __launch_bounds__(1024) step_kernel(const DomDev *__restrict__ doms, const int *__restrict__ tile_dom,
-                                                    int xcd_map) {
-    using SP = Storage<ST>;
-    using T = typename SP::T;
-    constexpr bool NTL = (VAR & kVarNtLoad) != 0;
-    constexpr bool NTS = (VAR & kVarNtStore) != 0;
-    const int nb = (int)gridDim.x;
-    const int t = xcd_map ? xcd_tile((int)blockIdx.x, nb) : (int)blockIdx.x;
-    const DomDev *D = doms + tile_dom[t];
-    const int cells = (int)D->cells;
-    const int block_first = (t - D->tile_begin) * (int)blockDim.x;
-    const int gi = block_first + (int)threadIdx.x;
-    if (gi >= cells) return;
-    const unsigned g = (unsigned)gi;
-    gchar *pl[19];
-    {
-        gchar *base = (gchar *)D->dd;
-        const size_t pb = (size_t)D->plane * SP::kBytes;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) pl[i] = base + (size_t)i * pb;
-    }
-    const int flag = ((const guchar *)D->flags)[g];
-
-    T f[19];
-    int d[19];
-    bool wrap = false;
-    if (BETA) {
-        const int X = D->nx, XY = D->nx * D->ny;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) d[i] = ex(i) + X * ey(i) + XY * ez(i);
-        const int dmax = 1 + X + XY;
-        wrap = block_first < dmax || block_first + (int)blockDim.x + dmax > cells;
-        if (wrap)
-            beta_gather<SP, true, NTL>(pl, g, cells, d, f);
-        else
-            beta_gather<SP, false, NTL>(pl, g, cells, d, f);
-    } else {
-        // the 19 loads do not wait for the flag; a GHOST cell (lbm_alpha.cl:30-32)
-        // matches no branch below and stores nothing
-#pragma unroll
-        for (int i = 0; i < 19; ++i) f[i] = SP::template ld<NTL>(pl[i], g * SP::kBytes, i);
-    }
-
-    T rho, vx, vy, vz;
+// Moments in the kernel's summation order, the flag branch and the collision
+// of one cell.  f[] holds the gathered (beta) or local (alpha) populations on
+// entry and the values to store on exit; returns whether they are stored
+// (alpha stores FLUID and VELOCITY_INJECTION cells only, beta every cell).
+template <typename T, bool BETA>
+__device__ __forceinline__ bool cell_update(T *f, int flag, const DomDev *D, T &vx, T &vy, T &vz, T &dens_out) {
+    T rho;
     if (BETA) {  // summation order of lbm_beta.cl:268-483
         rho = f[3]; vy = -f[3];
         rho = rho + f[2];  vy = vy + f[2];
@@ -439,8 +390,7 @@ __global__ void __launch_bounds__(1024) step_kernel(const DomDev *__restrict__ d
         rho = rho + f[17]; vz = vz - f[17];
         rho = rho + f[18];
     }
-
-    T dens_out = rho;
+    dens_out = rho;
     bool write_dd = BETA;
     if (flag == kFlagFluid) {
         const T dd_param = rho - T(1.5) * vel2_of(vx, vy, vz);
@@ -473,27 +423,160 @@ __global__ void __launch_bounds__(1024) step_kernel(const DomDev *__restrict__ d
         else dens_out = one;
         write_dd = true;
     }
+    return write_dd;
+}
 
+template <typename T>
+__device__ __forceinline__ void store_macros(const DomDev *D, unsigned g, int cells, T vx, T vy, T vz, T dens) {
+    typedef __attribute__((address_space(1))) T gT;
+    const unsigned cs = (unsigned)sizeof(T);
+    if (D->store_velocity) {
+        gchar *v = (gchar *)D->vel;
+        *(gT *)(v + g * cs) = vx;
+        *(gT *)(v + (size_t)cells * cs + g * cs) = vy;
+        *(gT *)(v + 2 * (size_t)cells * cs + g * cs) = vz;
+    }
+    if (D->store_density) *(gT *)((gchar *)D->dens + g * cs) = dens;
+}
+
+// One cell, element accesses.  With WRAP the neighbour index goes through the
+// linear wrap W (src/cl_programs/wrap.h:112-127); without it the block is far
+// enough from both ends of the domain for W to be the identity.
+template <class SP, bool BETA, bool NTS>
+__device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
+                                            const DomDev *D) {
+    using T = typename SP::T;
+    const int flag = ((const guchar *)D->flags)[g];
+    T f[19];
+    if (BETA) {
+#pragma unroll
+        for (int i = 0; i < 18; ++i) {
+            const unsigned idx = wrap ? (unsigned)wrap_idx((int)g - d[i], cells) : g - (unsigned)d[i];
+            f[i] = ld<SP>(pl[opp(i)], idx * SP::kBytes, opp(i));
+        }
+        f[18] = ld<SP>(pl[18], g * SP::kBytes, 18);
+    } else {
+        // the 19 loads do not wait for the flag; a GHOST cell (lbm_alpha.cl:30-32)
+        // matches no branch of cell_update and stores nothing
+#pragma unroll
+        for (int i = 0; i < 19; ++i) f[i] = ld<SP>(pl[i], g * SP::kBytes, i);
+    }
+    T vx, vy, vz, dens;
+    const bool write_dd = cell_update<T, BETA>(f, flag, D, vx, vy, vz, dens);
     if (write_dd) {
         if (BETA) {
-            if (wrap)
-                beta_scatter<SP, true, NTS>(pl, g, cells, d, f);
-            else
-                beta_scatter<SP, false, NTS>(pl, g, cells, d, f);
+#pragma unroll
+            for (int i = 0; i < 18; ++i) {
+                const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
+                st<SP, NTS>(pl[i], idx * SP::kBytes, i, f[i]);
+            }
+            st<SP, NTS>(pl[18], g * SP::kBytes, 18, f[18]);
         } else {
 #pragma unroll
-            for (int i = 0; i < 19; ++i) SP::template st<NTS>(pl[opp(i)], g * SP::kBytes, opp(i), f[i]);
+            for (int i = 0; i < 19; ++i) st<SP, NTS>(pl[opp(i)], g * SP::kBytes, opp(i), f[i]);
         }
     }
     if (flag == kFlagGhost) return;  // lbm_alpha.cl:30-32, lbm_beta.cl:787-788
-    if (D->store_velocity) {
-        gchar *v = (gchar *)D->vel;
-        const unsigned cs = (unsigned)sizeof(T);
-        *(__attribute__((address_space(1))) T *)(v + g * cs) = vx;
-        *(__attribute__((address_space(1))) T *)(v + (size_t)cells * cs + g * cs) = vy;
-        *(__attribute__((address_space(1))) T *)(v + 2 * (size_t)cells * cs + g * cs) = vz;
+    store_macros<T>(D, g, cells, vx, vy, vz, dens);
+}
+
+// N x-consecutive cells g0 .. g0+N-1 away from the domain ends: each of the
+// 19 (beta: 18 shifted + 1) plane accesses is one N-element vector access.
+template <class SP, bool BETA, int N>
+__device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int cells, const int *d,
+                                             const DomDev *D) {
+    using T = typename SP::T;
+    const unsigned es = SP::kBytes;
+    int flag[N];
+    {
+        typedef unsigned char VF __attribute__((ext_vector_type(N)));
+        VF fv;
+        __builtin_memcpy(&fv, (const gchar *)D->flags + g0, sizeof(VF));
+#pragma unroll
+        for (int j = 0; j < N; ++j) flag[j] = fv[j];
     }
-    if (D->store_density) *(__attribute__((address_space(1))) T *)((gchar *)D->dens + g * (unsigned)sizeof(T)) = dens_out;
+    T F[19][N];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const int src = BETA ? opp(i) : i;
+        const unsigned off = BETA ? (g0 - (unsigned)d[i]) * es : g0 * es;
+        vld<SP, N>(pl[src], off, src, F[i]);
+    }
+    T vx[N], vy[N], vz[N], dn[N];
+    bool all_write = true, any_ghost = false;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        T f[19];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) f[i] = F[i][j];
+        const bool w = cell_update<T, BETA>(f, flag[j], D, vx[j], vy[j], vz[j], dn[j]);
+        all_write = all_write && w;
+        any_ghost = any_ghost || flag[j] == kFlagGhost;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) F[i][j] = w ? f[i] : F[i][j];
+        if (!BETA && !w) flag[j] = -flag[j] - 1;  // remember: this cell stores nothing
+    }
+    if (BETA) {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) vst<SP, N>(pl[i], (g0 + (unsigned)d[i]) * es, i, F[i]);
+    } else if (all_write) {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) vst<SP, N>(pl[opp(i)], g0 * es, opp(i), F[i]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if (flag[j] >= 0)
+#pragma unroll
+                for (int i = 0; i < 19; ++i) st<SP>(pl[opp(i)], (g0 + j) * es, opp(i), F[i][j]);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int fl = flag[j] < 0 ? -flag[j] - 1 : flag[j];
+        if (fl != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx[j], vy[j], vz[j], dn[j]);
+    }
+    (void)any_ghost;
+}
+
+template <int ST, bool BETA, int VAR = 0>
+__global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ doms, const int *__restrict__ tile_dom,
+                                                    int xcd_map) {
+    using SP = Storage<ST>;
+    constexpr bool NTS = (VAR & kVarNtStore) != 0;
+    constexpr int N = cells_per_lane(VAR);
+    const int nb = (int)gridDim.x;
+    const int t = xcd_map ? xcd_tile((int)blockIdx.x, nb) : (int)blockIdx.x;
+    const DomDev *D = doms + tile_dom[t];
+    const int cells = (int)D->cells;
+    const int block_cells = (int)blockDim.x * N;
+    const int block_first = (t - D->tile_begin) * block_cells;
+    const int g0 = block_first + (int)threadIdx.x * N;
+    if (g0 >= cells) return;
+    gchar *pl[19];
+    {
+        gchar *base = (gchar *)D->dd;
+        const size_t pb = (size_t)D->plane * SP::kBytes;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) pl[i] = base + (size_t)i * pb;
+    }
+    int d[19];
+    bool wrap = false;
+    if (BETA) {
+        const int X = D->nx, XY = D->nx * D->ny;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) d[i] = ex(i) + X * ey(i) + XY * ez(i);
+        const int dmax = 1 + X + XY;
+        wrap = block_first < dmax || block_first + block_cells + dmax > cells;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) d[i] = 0;
+    }
+    if (N > 1 && !wrap && g0 + N <= cells) {
+        cells_vector<SP, BETA, N>(pl, (unsigned)g0, cells, d, D);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (g0 + j < cells) cell_scalar<SP, BETA, NTS>(pl, (unsigned)(g0 + j), cells, d, wrap, D);
 }
 
 }  // namespace lbm

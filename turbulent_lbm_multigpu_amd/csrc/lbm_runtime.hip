@@ -91,7 +91,7 @@ __global__ void init_kernel(DomDev D, InitArgs a) {
     T f[19];
     feq19(dd_param, u[0], u[1], u[2], f);
 #pragma unroll
-    for (int i = 0; i < 19; ++i) SP::template st<false>(plane_ptr<SP>(D, opp(i)), (unsigned)g * SP::kBytes, opp(i), f[i]);
+    for (int i = 0; i < 19; ++i) st<SP>(plane_ptr<SP>(D, opp(i)), (unsigned)g * SP::kBytes, opp(i), f[i]);
     if (D.vel) {
         T *v = (T *)D.vel;
         v[g] = u[0];
@@ -121,7 +121,7 @@ __global__ void pack_kernel(DomDev D, int field, Box b, void *out) {
     if (field == LBM_FIELD_DD) {
         T *o = (T *)out;
 #pragma unroll
-        for (int i = 0; i < 19; ++i) o[i * n + e] = SP::template ld<false>(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i);
+        for (int i = 0; i < 19; ++i) o[i * n + e] = ld<SP>(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i);
     } else if (field == LBM_FIELD_VELOCITY) {
         const T *v = (const T *)D.vel;
         T *o = (T *)out;
@@ -146,7 +146,7 @@ __global__ void unpack_kernel(DomDev D, int field, Box b, const void *in, uint32
         const T *s = (const T *)in;
 #pragma unroll
         for (int i = 0; i < 19; ++i)
-            if (mask & (1u << i)) SP::template st<false>(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i, s[i * n + e]);
+            if (mask & (1u << i)) st<SP>(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i, s[i * n + e]);
     } else if (field == LBM_FIELD_VELOCITY) {
         T *v = (T *)D.vel;
         const T *s = (const T *)in;
@@ -213,7 +213,7 @@ __device__ __forceinline__ void cell_moments(const DomDev &D, long long g, bool 
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const int slot = swapped ? opp(i) : i;
-        const double v = (double)SP::template ld<false>(plane_ptr<SP>(D, slot), (unsigned)g * SP::kBytes, slot);
+        const double v = (double)ld<SP>(plane_ptr<SP>(D, slot), (unsigned)g * SP::kBytes, slot);
         r += v;
         a += v * ex(i);
         bb += v * ey(i);
@@ -443,7 +443,7 @@ int upload_desc(lbm_solver *s) {
 hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
                               const int *tile_table, hipStream_t stream) {
     const int xcd = (variant & 4) ? 0 : 1;
-    const int v = variant & 3;
+    const int v = variant & ~4;
     dispatch_storage(storage, [&](auto st) -> int {
         constexpr int ST = decltype(st)::value;
         auto go = [&](auto b, auto vv) {
@@ -454,9 +454,10 @@ hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int 
         auto pick = [&](auto b) {
             switch (v) {
                 case 0: go(b, std::integral_constant<int, 0>()); break;
-                case 1: go(b, std::integral_constant<int, 1>()); break;
                 case 2: go(b, std::integral_constant<int, 2>()); break;
-                default: go(b, std::integral_constant<int, 3>()); break;
+                case 8: go(b, std::integral_constant<int, 8>()); break;
+                case 16: go(b, std::integral_constant<int, 16>()); break;
+                default: go(b, std::integral_constant<int, 0>()); break;
             }
         };
         if (beta) pick(std::true_type());
@@ -549,14 +550,19 @@ extern "C" int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out)
                       "one distribution plane must stay below 4 GiB: split the domain into subdomains");
     if (c.forcing != 0 && c.forcing != 1) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "unknown forcing");
     int wg = c.work_group_size == 0 ? 256 : c.work_group_size;
-    if (wg < 64 || wg > 1024 || wg % 64 != 0)
-        return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "work_group_size must be a multiple of 64 in [64, 1024]");
+    if (wg < 64 || wg > 256 || wg % 64 != 0)
+        return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "work_group_size must be 64, 128, 192 or 256");
     if (!(c.inv_tau > 0.0) || !std::isfinite(c.inv_tau))
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "inv_tau must be positive and finite");
     if (c.smagorinsky_cs < 0.0 || (c.smagorinsky_cs > 0.0 && !(c.tau > 0.5)))
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "LES needs smagorinsky_cs >= 0 and tau > 0.5");
-    if (c.kernel_variant < 0 || c.kernel_variant > 7 || c.lds_per_block < 0 || c.lds_per_block > 65536)
-        return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "kernel_variant must be in [0, 7], lds_per_block in [0, 65536]");
+    {
+        const int v = c.kernel_variant & ~4;
+        if (c.kernel_variant < 0 || (v != 0 && v != 2 && v != 8 && v != 16) || c.lds_per_block < 0 ||
+            c.lds_per_block > 65536)
+            return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
+                          "kernel_variant must be one of 0, 2, 8, 16 (optionally | 4); lds_per_block in [0, 65536]");
+    }
 
     lbm_solver *s = new lbm_solver();
     s->cfg = c;
@@ -565,7 +571,10 @@ extern "C" int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out)
     s->cells = cells;
     const long long pad = c.plane_padding < 0 ? 0 : c.plane_padding;
     s->plane = ((cells + 255) / 256) * 256 + pad;
-    s->ntiles = (int)((cells + wg - 1) / wg);
+    {
+        const long long per_tile = (long long)wg * cells_per_lane(c.kernel_variant);
+        s->ntiles = (int)((cells + per_tile - 1) / per_tile);
+    }
     auto bail = [&](int code, const std::string &m) {
         record(nullptr, code, m);
         lbm_solver_destroy(s);
@@ -1037,8 +1046,11 @@ extern "C" int lbm_group_create(lbm_solver *const *solvers, int n, lbm_group **o
         if (!solvers[i]) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null member solver");
         if (solvers[i]->group) return record(nullptr, LBM_ERR_STATE, "a solver can belong to one group only");
         if (solvers[i]->cfg.storage != solvers[0]->cfg.storage || solvers[i]->wg != solvers[0]->wg ||
-            solvers[i]->cfg.device != solvers[0]->cfg.device)
-            return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "group members need the same storage, work-group size and device");
+            solvers[i]->cfg.device != solvers[0]->cfg.device ||
+            solvers[i]->cfg.kernel_variant != solvers[0]->cfg.kernel_variant ||
+            solvers[i]->cfg.lds_per_block != solvers[0]->cfg.lds_per_block)
+            return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
+                          "group members need the same storage, work-group size, kernel variant and device");
         for (int j = 0; j < i; ++j)
             if (solvers[j] == solvers[i]) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "duplicate member solver");
     }
