@@ -148,6 +148,11 @@ int lbm_solver_moments(lbm_solver *s, const int origin[3], const int size[3], vo
  * src/CController.hpp:265-383); norm == NULL copies all 19 populations */
 int lbm_copy_box(lbm_solver *src, const int src_origin[3], lbm_solver *dst, const int dst_origin[3],
                  const int size[3], const int norm[3]);
+/* seeded spherical obstacles (new): spheres with centres uniform in the key
+ * grid's interior and radii in [rmin, rmax] are drawn until they cover `fill`
+ * of it; FLUID cells inside become OBSTACLE.  achieved = covered fraction. */
+int lbm_solver_place_spheres(lbm_solver *s, uint64_t seed, double fill, double rmin, double rmax,
+                             const int key_origin[3], const int key_dims[3], double *achieved);
 /* raw device pointers (for callers that bring their own kernels) */
 int lbm_solver_device_pointers(lbm_solver *s, void **dd, void **flags, long long *plane_stride);
 
@@ -179,6 +184,16 @@ int lbm_group_wait(lbm_group *g);
 int lbm_group_set_timing(lbm_group *g, int enable);
 int lbm_group_get_timing(lbm_group *g, double *step_kernel_ms, long long *step_kernel_launches,
                          double *exchange_ms, long long *exchange_launches);
+/* launch shape of the fused step kernel: work-group size (64..256) and
+ * kernel variant (see lbm_solver_config.kernel_variant); results do not
+ * depend on it */
+int lbm_group_set_launch(lbm_group *g, int work_group_size, int kernel_variant);
+int lbm_group_get_launch(const lbm_group *g, int *work_group_size, int *kernel_variant);
+/* autotuner (new): times `pairs` step pairs for each work-group size in
+ * {64,128,256} x cells-per-lane in {1,2,4}, keeps the fastest; the timed steps
+ * are real steps of the simulation (18 * (pairs + 1) of them, plus one if the
+ * counter was odd) */
+int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step);
 /* the same averages split into beta and alpha step kernels */
 int lbm_group_get_timing_detail(lbm_group *g, double *beta_ms, double *alpha_ms, double *exchange_ms);
 

@@ -1,0 +1,72 @@
+"""Reduced-precision storage and the channel configuration on the GPU.
+
+* body force: on a lattice without walls the total momentum grows by exactly
+  rho*g per cell and step (to FP64 rounding), pinning the forcing term;
+* periodic channel with seeded spheres: FP16(f - w) and BF16(f - w) storage
+  against the same run in FP64, with the error bounds stated below;
+* FP32 storage against FP64 as the yardstick of what arithmetic alone costs.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from turbulent_lbm_multigpu_amd import _capi  # noqa: E402
+from turbulent_lbm_multigpu_amd import precision  # noqa: E402
+from turbulent_lbm_multigpu_amd.solver import LbmSolver  # noqa: E402
+
+# Stated bounds for the reduced channel (96x48x48, 2000 steps): rho as an
+# absolute error, u relative to the largest FP64 speed U over fluid cells, ke
+# relative.  Measured on MI355X (deterministic): f32 rho 6.4e-5 / u 5.8e-3 /
+# ke 1.1e-2, f16 8.1e-4 / 5.3e-3 / 9.9e-3, bf16 6.9e-4 / 1.0e-2 / 7.5e-3.  The
+# FP32 collision dominates: the per-step body-force increment 3 w g ~ 1.7e-7 is
+# only ~45 ulp of an FP32 population ~0.055, so every storage that collides in
+# FP32 drifts from FP64 by ~1% in kinetic energy after 2000 steps; the shifted
+# FP16/BF16 storage adds little on top (u_rms within 1.5x of FP32's).
+BOUNDS = {
+    "f32": dict(rho=2e-4, u=1.5e-2, ke=3e-2),
+    "f16": dict(rho=2.5e-3, u=1.5e-2, ke=3e-2),
+    "bf16": dict(rho=2.5e-3, u=3e-2, ke=3e-2),
+}
+RMS_VS_F32 = 1.5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_body_force_momentum_balance_fp64():
+    say("case: body force momentum balance 16x12x10 fp64")
+    g = np.array([1e-5, -3e-6, 2e-6])
+    s = LbmSolver((16, 12, 10), (2,) * 6, inv_tau=1 / 0.8, gravitation=g, forcing=_capi.FORCING_BODY,
+                  storage=_capi.STORAGE_F64)
+    s.init_perturbed(1e-3, 11)
+    rho0, j0 = s.moments()
+    steps = 40
+    s.run(steps)
+    rho1, j1 = s.moments()
+    m0 = rho0.sum()
+    assert abs(rho1.sum() - m0) < 1e-10
+    gained = j1.reshape(3, -1).sum(1) - j0.reshape(3, -1).sum(1)
+    # each step adds rho_cell * g per cell; rho of each cell drifts only at O(1e-3)
+    np.testing.assert_allclose(gained, steps * m0 * g, rtol=5e-3)
+    s.close()
+
+
+def test_channel_storage_errors_vs_fp64():
+    say("case: channel 96x48x48 2000 steps, f64/f32/f16/bf16")
+    res = precision.run((96, 48, 48), 2000, ("f64", "f32", "f16", "bf16"))
+    say(f"  {res}")
+    assert abs(res["f64"]["mass_end"] - res["f64"]["mass_start"]) / res["f64"]["mass_start"] < 1e-3
+    assert 0.045 < res["f64"]["obstacle_fill"] < 0.06
+    for name, b in BOUNDS.items():
+        r = res[name]
+        assert r["rho_max_abs_err"] < b["rho"], (name, r)
+        assert r["u_max_abs_err_over_U"] < b["u"], (name, r)
+        assert r["ke_rel_err"] < b["ke"], (name, r)
+        assert r["u_rms_err_over_U"] <= RMS_VS_F32 * res["f32"]["u_rms_err_over_U"], (name, r)
+    # the flow is driven: the largest speed is far above the initial noise
+    assert res["f32"]["U_ref"] > 1e-4

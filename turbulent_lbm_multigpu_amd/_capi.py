@@ -52,7 +52,8 @@ EXPORTED_SYMBOLS = (
     "lbm_solver_device_pointers", "lbm_group_create", "lbm_group_destroy", "lbm_group_error",
     "lbm_group_add_comm", "lbm_group_set_exchange_mode", "lbm_group_set_graph", "lbm_group_step",
     "lbm_group_wait", "lbm_group_set_timing", "lbm_group_get_timing", "lbm_group_sync",
-    "lbm_group_get_timing_detail",
+    "lbm_group_get_timing_detail", "lbm_solver_place_spheres", "lbm_group_set_launch",
+    "lbm_group_get_launch", "lbm_group_autotune",
 )
 
 
@@ -137,6 +138,7 @@ def load_library(path: str | None = None):
     sig("lbm_solver_moments", I, P, IP, IP, P, P)
     sig("lbm_copy_box", I, P, IP, P, IP, IP, IP)
     sig("lbm_solver_device_pointers", I, P, C.POINTER(P), C.POINTER(P), C.POINTER(C.c_longlong))
+    sig("lbm_solver_place_spheres", I, P, C.c_uint64, D, D, D, IP, IP, C.POINTER(D))
     sig("lbm_group_create", I, C.POINTER(P), I, C.POINTER(P))
     sig("lbm_group_destroy", I, P)
     sig("lbm_group_error", C.c_char_p, P)
@@ -147,6 +149,9 @@ def load_library(path: str | None = None):
     sig("lbm_group_wait", I, P)
     sig("lbm_group_sync", I, P, I)
     sig("lbm_group_get_timing_detail", I, P, C.POINTER(D), C.POINTER(D), C.POINTER(D))
+    sig("lbm_group_set_launch", I, P, I, I)
+    sig("lbm_group_get_launch", I, P, IP, IP)
+    sig("lbm_group_autotune", I, P, I, IP, IP, C.POINTER(D))
     sig("lbm_group_set_timing", I, P, I)
     sig("lbm_group_get_timing", I, P, C.POINTER(D), C.POINTER(C.c_longlong), C.POINTER(D),
         C.POINTER(C.c_longlong))

@@ -351,6 +351,7 @@ struct lbm_group {
     std::vector<hipStream_t> saved_streams;
     hipStream_t stream = nullptr;
     int storage = 0, wg = 256, device = 0;
+    int variant = 0, lds = 0;  // launch shape of the fused step kernel (tuner-selected)
     std::string err;
     DomDev *d_descs = nullptr;
     int *d_tiles = nullptr;
@@ -934,6 +935,96 @@ extern "C" int lbm_copy_box(lbm_solver *src, const int src_origin[3], lbm_solver
     return LBM_OK;
 }
 
+// ---- seeded spherical obstacles ------------------------------------------------
+
+namespace {
+uint64_t host_mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double host_unit(uint64_t seed, uint64_t key) {
+    return (double)(host_mix64(seed ^ host_mix64(key)) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void mark_obstacles_kernel(DomDev D, const uint8_t *solid, Box b) {
+    const long long n = (long long)b.s[0] * b.s[1] * b.s[2];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n || !solid[e]) return;
+    const long long g = box_cell(D, b, e);
+    const uint8_t f = D.flags[g];
+    if (f == kFlagFluid) D.flags[g] = (uint8_t)kFlagObstacle;
+}
+}  // namespace
+
+// Spheres with centres uniform in the key grid's interior and radii uniform in
+// [rmin, rmax], drawn from splitmix64(seed, k) in order k = 0, 1, ... until the
+// union covers `fill` of the key grid's interior cells; every FLUID cell of the
+// solver inside a sphere (at its key-grid position) becomes OBSTACLE.  The
+// same (seed, key grid) gives the same obstacles in every subdomain.
+extern "C" int lbm_solver_place_spheres(lbm_solver *s, uint64_t seed, double fill, double rmin, double rmax,
+                                        const int key_origin[3], const int key_dims[3], double *achieved) {
+    if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
+    if (!(fill >= 0.0 && fill < 0.9) || !(rmin > 0.0) || !(rmax >= rmin))
+        return fail(s, LBM_ERR_INVALID_ARGUMENT, "need 0 <= fill < 0.9 and 0 < rmin <= rmax");
+    int ko[3], kd[3];
+    for (int a = 0; a < 3; ++a) {
+        ko[a] = key_origin ? key_origin[a] : 0;
+        kd[a] = key_dims ? key_dims[a] : s->cfg.dims[a];
+        if (kd[a] < 3 || ko[a] < 0 || ko[a] + s->cfg.dims[a] > kd[a])
+            return fail(s, LBM_ERR_OUT_OF_RANGE, "key grid must contain the domain");
+    }
+    const long long KX = kd[0], KY = kd[1], KZ = kd[2];
+    const long long interior = (KX - 2) * (KY - 2) * (KZ - 2);
+    if (KX * KY * KZ > (1ll << 33)) return fail(s, LBM_ERR_INVALID_ARGUMENT, "key grid too large");
+    std::vector<uint8_t> solid((size_t)(KX * KY * KZ), 0);
+    long long marked = 0, k = 0;
+    const long long target = (long long)std::ceil(fill * (double)interior);
+    while (marked < target && k < 4000000) {
+        const double cx = 1.0 + host_unit(seed, 4 * k + 0) * (double)(KX - 2);
+        const double cy = 1.0 + host_unit(seed, 4 * k + 1) * (double)(KY - 2);
+        const double cz = 1.0 + host_unit(seed, 4 * k + 2) * (double)(KZ - 2);
+        const double r = rmin + host_unit(seed, 4 * k + 3) * (rmax - rmin);
+        ++k;
+        const long long x0 = std::max(1ll, (long long)std::floor(cx - r)), x1 = std::min(KX - 2, (long long)std::ceil(cx + r));
+        const long long y0 = std::max(1ll, (long long)std::floor(cy - r)), y1 = std::min(KY - 2, (long long)std::ceil(cy + r));
+        const long long z0 = std::max(1ll, (long long)std::floor(cz - r)), z1 = std::min(KZ - 2, (long long)std::ceil(cz + r));
+        for (long long z = z0; z <= z1; ++z)
+            for (long long y = y0; y <= y1; ++y)
+                for (long long x = x0; x <= x1; ++x) {
+                    const double dx = x - cx, dy = y - cy, dz = z - cz;
+                    if (dx * dx + dy * dy + dz * dz > r * r) continue;
+                    uint8_t &c = solid[(size_t)(x + KX * (y + KY * z))];
+                    if (!c) {
+                        c = 1;
+                        ++marked;
+                    }
+                }
+    }
+    if (achieved) *achieved = interior ? (double)marked / (double)interior : 0.0;
+    // the solver's box inside the key grid
+    Box b;
+    for (int a = 0; a < 3; ++a) {
+        b.o[a] = 0;
+        b.s[a] = s->cfg.dims[a];
+    }
+    const long long n = s->cells;
+    std::vector<uint8_t> local((size_t)n);
+    for (int z = 0; z < s->cfg.dims[2]; ++z)
+        for (int y = 0; y < s->cfg.dims[1]; ++y)
+            for (int x = 0; x < s->cfg.dims[0]; ++x)
+                local[(size_t)(x + (long long)s->cfg.dims[0] * (y + (long long)s->cfg.dims[1] * z))] =
+                    solid[(size_t)((ko[0] + x) + KX * ((ko[1] + y) + KY * (long long)(ko[2] + z)))];
+    int rc = ensure_staging(s, (size_t)n);
+    if (rc) return rc;
+    SCHK(hipMemcpyAsync(s->staging, local.data(), (size_t)n, hipMemcpyHostToDevice, s->stream));
+    mark_obstacles_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(s->host_desc, (const uint8_t *)s->staging, b);
+    SCHK(hipGetLastError());
+    SCHK(hipStreamSynchronize(s->stream));
+    return LBM_OK;
+}
+
 // ---- device copy bandwidth (calibration of the roofline) --------------------
 
 __global__ void __launch_bounds__(256) stream_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
@@ -1023,16 +1114,29 @@ static void group_free_plan(lbm_group *g) {
     g->plan_built = false;
 }
 
+// Descriptor and tile tables of the fused launch for the group's current
+// launch shape (work-group size and cells per lane).
 static int group_upload_descs(lbm_group *g) {
     std::vector<DomDev> descs;
-    int tile = 0;
-    for (auto *m : g->members) {
+    std::vector<int> tiles;
+    const long long per_tile = (long long)g->wg * cells_per_lane(g->variant);
+    for (size_t i = 0; i < g->members.size(); ++i) {
+        lbm_solver *m = g->members[i];
         fill_desc(m);
         DomDev d = m->host_desc;
-        d.tile_begin = tile;
+        const int nt = (int)((m->cells + per_tile - 1) / per_tile);
+        d.tile_begin = (int)tiles.size();
+        d.ntiles = nt;
         descs.push_back(d);
-        tile += m->ntiles;
+        tiles.insert(tiles.end(), (size_t)nt, (int)i);
     }
+    if ((int)tiles.size() != g->ntiles || !g->d_tiles) {
+        if (g->d_tiles) GCHK(hipFree(g->d_tiles));
+        g->d_tiles = nullptr;
+        GCHK(hipMalloc((void **)&g->d_tiles, sizeof(int) * tiles.size()));
+        g->ntiles = (int)tiles.size();
+    }
+    GCHK(hipMemcpyAsync(g->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, g->stream));
     GCHK(hipMemcpyAsync(g->d_descs, descs.data(), sizeof(DomDev) * descs.size(), hipMemcpyHostToDevice, g->stream));
     GCHK(hipStreamSynchronize(g->stream));
     for (size_t i = 0; i < g->members.size(); ++i) g->seen_versions[i] = g->members[i]->version;
@@ -1058,10 +1162,9 @@ extern "C" int lbm_group_create(lbm_solver *const *solvers, int n, lbm_group **o
     g->storage = solvers[0]->cfg.storage;
     g->wg = solvers[0]->wg;
     g->device = solvers[0]->cfg.device;
-    for (int i = 0; i < n; ++i) {
-        g->members.push_back(solvers[i]);
-        g->ntiles += solvers[i]->ntiles;
-    }
+    g->variant = solvers[0]->cfg.kernel_variant;
+    g->lds = solvers[0]->cfg.lds_per_block;
+    for (int i = 0; i < n; ++i) g->members.push_back(solvers[i]);
     g->seen_versions.assign(n, ~0u);
     auto bail = [&](int code, const std::string &m) {
         record(nullptr, code, m);
@@ -1074,20 +1177,14 @@ extern "C" int lbm_group_create(lbm_solver *const *solvers, int n, lbm_group **o
         if (hipStreamSynchronize(m->stream) != hipSuccess) return bail(LBM_ERR_DEVICE, "member stream sync failed");
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(LBM_ERR_DEVICE, "hipStreamCreate failed");
-    if (hipMalloc((void **)&g->d_descs, sizeof(DomDev) * n) != hipSuccess ||
-        hipMalloc((void **)&g->d_tiles, sizeof(int) * (size_t)g->ntiles) != hipSuccess)
+    if (hipMalloc((void **)&g->d_descs, sizeof(DomDev) * n) != hipSuccess)
         return bail(LBM_ERR_OUT_OF_MEMORY, "group table allocation failed");
-    std::vector<int> tiles((size_t)g->ntiles);
-    {
-        int t = 0;
-        for (int i = 0; i < n; ++i)
-            for (int k = 0; k < solvers[i]->ntiles; ++k) tiles[t++] = i;
-    }
-    if (hipMemcpy(g->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess)
-        return bail(LBM_ERR_DEVICE, "tile table upload failed");
+    // members advance together, so their step counters must share a parity
+    // (the AA pattern's layout); each keeps its own count
     g->step_counter = solvers[0]->step_counter;
     for (auto *m : g->members) {
-        if (m->step_counter != g->step_counter) return bail(LBM_ERR_STATE, "member step counters differ");
+        if ((m->step_counter & 1) != (g->step_counter & 1))
+            return bail(LBM_ERR_STATE, "member step counters differ in parity");
         g->saved_streams.push_back(m->stream);
         m->stream = g->stream;
         m->group = g;
@@ -1214,9 +1311,8 @@ static int group_launch_half(lbm_group *g, bool beta, bool timed) {
         GCHK(hipEventCreate(&eb));
         GCHK(hipEventRecord(ea, g->stream));
     }
-    const lbm_solver_config &c0 = g->members[0]->cfg;
-    GCHK(launch_step_kernel(g->storage, beta, c0.kernel_variant, c0.lds_per_block, g->ntiles, g->wg, g->d_descs,
-                            g->d_tiles, g->stream));
+    GCHK(launch_step_kernel(g->storage, beta, g->variant, g->lds, g->ntiles, g->wg, g->d_descs, g->d_tiles,
+                            g->stream));
     if (timed) {
         GCHK(hipEventRecord(eb, g->stream));
         g->events.push_back({ea, eb, beta ? 0 : 2});
@@ -1282,7 +1378,90 @@ extern "C" int lbm_group_step(lbm_group *g, int steps) {
             done += 1;
         }
     }
-    for (auto *m : g->members) m->step_counter = g->step_counter;
+    for (auto *m : g->members) m->step_counter += done;
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_set_launch(lbm_group *g, int work_group_size, int kernel_variant) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    const int v = kernel_variant & ~4;
+    if (work_group_size < 64 || work_group_size > 256 || work_group_size % 64 != 0 || kernel_variant < 0 ||
+        (v != 0 && v != 2 && v != 8 && v != 16))
+        return gfail(g, LBM_ERR_INVALID_ARGUMENT, "unsupported launch shape");
+    GCHK(hipStreamSynchronize(g->stream));
+    g->wg = work_group_size;
+    g->variant = kernel_variant;
+    if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->graph_exec = nullptr;
+    g->graph = nullptr;
+    return group_upload_descs(g);
+}
+
+extern "C" int lbm_group_get_launch(const lbm_group *g, int *work_group_size, int *kernel_variant) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    if (work_group_size) *work_group_size = g->wg;
+    if (kernel_variant) *kernel_variant = g->variant;
+    return LBM_OK;
+}
+
+// Autotuner: times `pairs` beta+alpha step pairs (with their exchanges) for
+// every work-group size in {64, 128, 256} and every variant in {0, 8, 16}
+// (1, 2, 4 cells per lane) and keeps the fastest.  The steps it runs are
+// ordinary simulation steps: every launch shape computes the same
+// arithmetic, so the state is the one the same number of steps gives with
+// any shape.
+extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step) {
+    if (!g || pairs <= 0) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad autotune arguments");
+    GCHK(hipSetDevice(g->device));
+    if ((g->step_counter & 1) != 0) {
+        int rc = lbm_group_step(g, 1);  // start the pairs on a beta step
+        if (rc) return rc;
+    }
+    if (!g->plan_built) {
+        int rc = build_plan(g);
+        if (rc) return rc;
+    }
+    const int wgs[3] = {64, 128, 256};
+    const int vars[3] = {0, 8, 16};
+    double best = 1e300;
+    int bw = g->wg, bv = g->variant;
+    hipEvent_t e0, e1;
+    GCHK(hipEventCreate(&e0));
+    GCHK(hipEventCreate(&e1));
+    for (int wi = 0; wi < 3; ++wi)
+        for (int vi = 0; vi < 3; ++vi) {
+            int rc = lbm_group_set_launch(g, wgs[wi], vars[vi] | (g->variant & 4));
+            if (rc) return rc;
+            rc = group_launch_half(g, true, false);                 // warm-up pair
+            if (!rc) rc = group_launch_half(g, false, false);
+            if (rc) return rc;
+            GCHK(hipEventRecord(e0, g->stream));
+            for (int p = 0; p < pairs && !rc; ++p) {
+                rc = group_launch_half(g, true, false);
+                if (!rc) rc = group_launch_half(g, false, false);
+            }
+            if (rc) return rc;
+            GCHK(hipEventRecord(e1, g->stream));
+            GCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            GCHK(hipEventElapsedTime(&ms, e0, e1));
+            g->step_counter += 2 * (pairs + 1);
+            for (auto *m : g->members) m->step_counter += 2 * (pairs + 1);
+            const double per = (double)ms / (2.0 * pairs);
+            if (per < best) {
+                best = per;
+                bw = wgs[wi];
+                bv = vars[vi] | (g->variant & 4);
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int rc = lbm_group_set_launch(g, bw, bv);
+    if (rc) return rc;
+    if (best_wg) *best_wg = bw;
+    if (best_variant) *best_variant = bv;
+    if (best_ms_per_step) *best_ms_per_step = best;
     return LBM_OK;
 }
 

@@ -180,6 +180,17 @@ class LbmSolver:
         self._check(self._lib.lbm_solver_set(self._h, int(field), o, s, arr.ctypes.data_as(C.c_void_p),
                                              ivec(norm)))
 
+    def place_spheres(self, seed: int, fill: float, rmin: float, rmax: float, key_origin=(0, 0, 0),
+                      key_dims=None) -> float:
+        """Seeded spherical obstacles covering `fill` of the key grid interior;
+        returns the covered fraction."""
+        a = C.c_double()
+        self._check(self._lib.lbm_solver_place_spheres(self._h, C.c_uint64(seed), float(fill), float(rmin),
+                                                       float(rmax), ivec(key_origin),
+                                                       ivec(key_dims if key_dims is not None else self.dims),
+                                                       C.byref(a)))
+        return a.value
+
     def velocity_checksum(self) -> float:
         v = C.c_float()
         self._check(self._lib.lbm_solver_velocity_checksum(self._h, C.byref(v)))
@@ -238,6 +249,20 @@ class LbmGroup:
 
     def step(self, steps: int = 1):
         self._check(self._lib.lbm_group_step(self._h, int(steps)))
+
+    def set_launch(self, work_group_size: int, kernel_variant: int):
+        self._check(self._lib.lbm_group_set_launch(self._h, int(work_group_size), int(kernel_variant)))
+
+    def launch(self):
+        w, v = C.c_int(), C.c_int()
+        self._check(self._lib.lbm_group_get_launch(self._h, C.byref(w), C.byref(v)))
+        return w.value, v.value
+
+    def autotune(self, pairs: int = 4):
+        """Pick the fastest launch shape; the timed steps are simulation steps."""
+        w, v, ms = C.c_int(), C.c_int(), C.c_double()
+        self._check(self._lib.lbm_group_autotune(self._h, int(pairs), C.byref(w), C.byref(v), C.byref(ms)))
+        return {"work_group_size": w.value, "kernel_variant": v.value, "ms_per_step": ms.value}
 
     def wait(self):
         self._check(self._lib.lbm_group_wait(self._h))
