@@ -104,6 +104,18 @@ int lbm_device_copy_bandwidth(int device, size_t bytes, int reps, int variant, d
 /* constructor + reload() + reset() (src/CLbmSolver.hpp:221-260, 272-635) */
 int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out);
 int lbm_solver_destroy(lbm_solver *s);
+/* device bytes of a solver built by lbm_solver_create_in: distributions,
+ * flags, optional velocity/density, launch tables and a staging area of
+ * max(1 byte per cell, 19 compute values per cell of the largest face)
+ * (new; the reference allocates per buffer, src/CLbmSolver.hpp:381-403) */
+int lbm_solver_required_bytes(const lbm_solver_config *cfg, size_t *bytes);
+/* constructor over caller-owned device memory (256-byte aligned, at least
+ * lbm_solver_required_bytes): no device allocation and no free on destroy,
+ * so a resident case set can live in one arena.  `stream` is the hipStream_t
+ * the solver works on, NULL for a stream of its own.  Region transfers larger
+ * than the staging area fail with LBM_ERR_CAPACITY (new) */
+int lbm_solver_create_in(const lbm_solver_config *cfg, void *device_memory, size_t bytes, void *stream,
+                         lbm_solver **out);
 /* CError text of this solver (src/lib/CError.hpp:42-80) */
 const char *lbm_solver_error(const lbm_solver *s);
 /* reset(): init kernel, step counter 0 (src/CLbmSolver.hpp:619-635) */
@@ -196,6 +208,104 @@ int lbm_group_get_launch(const lbm_group *g, int *work_group_size, int *kernel_v
 int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step);
 /* the same averages split into beta and alpha step kernels */
 int lbm_group_get_timing_detail(lbm_group *g, double *beta_ms, double *alpha_ms, double *exchange_ms);
+/* Membership of a continuous batch (new; no reference counterpart): a group
+ * with no members yet; members join between steps when their step counter has
+ * the group's parity and leave with their state in place.  A member given a
+ * stop freezes after `steps` more half-steps of the group: the step and
+ * exchange kernels skip it from then on (a device-side half-step counter
+ * decides, so chunks of many steps run as one graph replay) and its own step
+ * counter stops there; steps < 0 removes the stop.  Removing a member
+ * renumbers the later ones and drops the comms that name it. */
+int lbm_group_create_empty(int device, int storage, int work_group_size, int kernel_variant, lbm_group **out);
+int lbm_group_add_member(lbm_group *g, lbm_solver *s, int *index);
+int lbm_group_remove_member(lbm_group *g, int index);
+int lbm_group_member_count(const lbm_group *g, int *count);
+int lbm_group_set_member_stop(lbm_group *g, int index, long long steps);
+/* the hipStream_t all members work on while in the group (for events) */
+int lbm_group_get_stream(const lbm_group *g, void **stream);
+
+/* ---- server: continuous batching of independent cases (new) ------------- */
+/* Cases (cavities, periodic channels) are admitted against a device-memory
+ * budget into one arena, stepped together per storage type as members of a
+ * group (one fused launch per half-step, graph-replayed chunks), frozen at
+ * their own step count and read back while the next chunk runs.  Requests
+ * that arrive while others run join at the next chunk boundary.  A case's
+ * result does not depend on what it was batched with: the per-cell
+ * arithmetic is the same in every launch shape and membership. */
+typedef struct lbm_server lbm_server;
+
+#define LBM_CASE_CAVITY 0  /* walls on all faces, lid layer at y = ny-2 moving in x */
+#define LBM_CASE_CHANNEL 1 /* periodic in x, walls in y and z, body force g_x, seeded spheres */
+
+#define LBM_CASE_QUEUED 0
+#define LBM_CASE_RUNNING 1
+#define LBM_CASE_DONE 2
+
+typedef struct lbm_server_config {
+    int device;
+    size_t memory_budget;  /* device bytes for case state (one arena) */
+    int max_active;        /* cases resident at once, 0 = limited by memory only */
+    double chunk_ms;       /* target device time of one scheduling chunk */
+    int max_chunk_steps;   /* steps per chunk at most (rounded to even) */
+    int work_group_size;   /* launch shape of the fused kernels */
+    int kernel_variant;
+    int autotune;          /* 1: tune the launch shape per storage on its first chunk */
+    int backfill;          /* 1: admit later requests that fit when the head of the queue does not */
+} lbm_server_config;
+
+typedef struct lbm_case {
+    int kind;                /* LBM_CASE_* */
+    int dims[3];             /* cells per axis including boundary/ghost layers */
+    int steps;
+    int storage;             /* LBM_STORAGE_*; results are float for every storage */
+    double tau;              /* BGK relaxation time (tau0 of the LES rate) */
+    double smagorinsky_cs;
+    double lid_velocity;     /* cavity lid speed */
+    double body_force;       /* channel g_x */
+    double obstacle_fill;    /* channel: sphere volume fraction */
+    double r_min, r_max;     /* channel: sphere radii */
+    double perturbation;     /* amplitude of the seeded initial velocity perturbation (0: rest) */
+    uint64_t seed;
+} lbm_case;
+
+typedef struct lbm_case_status {
+    long long id;
+    int state;             /* LBM_CASE_* state */
+    long long steps_done;
+    long long cells;
+    double submit_s, admit_s, done_s;  /* server clock (lbm_server_time) */
+} lbm_case_status;
+
+typedef struct lbm_server_stats {
+    long long submitted, completed, chunks, steps_cells;  /* steps_cells: sum of cells x steps run */
+    size_t bytes_in_use, bytes_peak;
+    int queued, active, ready;
+    double device_ms;  /* sum over chunks of the device time of the fused launches */
+} lbm_server_stats;
+
+void lbm_server_config_default(lbm_server_config *cfg);
+void lbm_case_default(lbm_case *c);
+int lbm_server_create(const lbm_server_config *cfg, lbm_server **out);
+int lbm_server_destroy(lbm_server *s);
+const char *lbm_server_error(const lbm_server *s);
+/* device bytes a case occupies while resident */
+int lbm_server_case_bytes(const lbm_case *c, size_t *bytes);
+/* queue a case; fails if it can never fit the budget */
+int lbm_server_submit(lbm_server *s, const lbm_case *c, long long *id);
+/* one scheduling round: wait for the chunk in flight, retire frozen cases
+ * and add admitted ones, launch the next chunk, then (while it runs) read
+ * back the retired cases and admit queued ones.  *completed = cases whose
+ * results became available in this call. */
+int lbm_server_progress(lbm_server *s, int *completed);
+/* oldest completed case not yet returned by this call (LBM_ERR_STATE: none) */
+int lbm_server_next_done(lbm_server *s, lbm_case_status *st);
+int lbm_server_status(const lbm_server *s, long long id, lbm_case_status *st);
+/* copy a completed case's rho[cells] and j[3][cells] (float, cell order
+ * x + X*(y + Y*z), from the distributions after its last step) and release
+ * them; null pointers only release */
+int lbm_server_fetch(lbm_server *s, long long id, float *rho, float *j);
+int lbm_server_stats_get(const lbm_server *s, lbm_server_stats *st);
+double lbm_server_time(const lbm_server *s);
 
 #ifdef __cplusplus
 }

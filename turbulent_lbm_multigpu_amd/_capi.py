@@ -40,6 +40,12 @@ FIELD_VELOCITY = 1
 FIELD_DENSITY = 2
 FIELD_FLAGS = 3
 
+CASE_CAVITY = 0
+CASE_CHANNEL = 1
+CASE_QUEUED = 0
+CASE_RUNNING = 1
+CASE_DONE = 2
+
 EXPORTED_SYMBOLS = (
     "lbm_version", "lbm_last_error", "lbm_device_synchronize", "lbm_solver_config_default",
     "lbm_device_copy_bandwidth",
@@ -54,6 +60,11 @@ EXPORTED_SYMBOLS = (
     "lbm_group_wait", "lbm_group_set_timing", "lbm_group_get_timing", "lbm_group_sync",
     "lbm_group_get_timing_detail", "lbm_solver_place_spheres", "lbm_group_set_launch",
     "lbm_group_get_launch", "lbm_group_autotune",
+    "lbm_solver_required_bytes", "lbm_solver_create_in", "lbm_group_create_empty", "lbm_group_add_member",
+    "lbm_group_remove_member", "lbm_group_member_count", "lbm_group_set_member_stop", "lbm_group_get_stream",
+    "lbm_server_config_default", "lbm_case_default", "lbm_server_create", "lbm_server_destroy",
+    "lbm_server_error", "lbm_server_case_bytes", "lbm_server_submit", "lbm_server_progress",
+    "lbm_server_next_done", "lbm_server_status", "lbm_server_fetch", "lbm_server_stats_get", "lbm_server_time",
 )
 
 
@@ -83,6 +94,65 @@ class SolverConfig(C.Structure):
         ("device", C.c_int),
         ("kernel_variant", C.c_int),
         ("lds_per_block", C.c_int),
+    ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int),
+        ("memory_budget", C.c_size_t),
+        ("max_active", C.c_int),
+        ("chunk_ms", C.c_double),
+        ("max_chunk_steps", C.c_int),
+        ("work_group_size", C.c_int),
+        ("kernel_variant", C.c_int),
+        ("autotune", C.c_int),
+        ("backfill", C.c_int),
+    ]
+
+
+class Case(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("dims", C.c_int * 3),
+        ("steps", C.c_int),
+        ("storage", C.c_int),
+        ("tau", C.c_double),
+        ("smagorinsky_cs", C.c_double),
+        ("lid_velocity", C.c_double),
+        ("body_force", C.c_double),
+        ("obstacle_fill", C.c_double),
+        ("r_min", C.c_double),
+        ("r_max", C.c_double),
+        ("perturbation", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class CaseStatus(C.Structure):
+    _fields_ = [
+        ("id", C.c_longlong),
+        ("state", C.c_int),
+        ("steps_done", C.c_longlong),
+        ("cells", C.c_longlong),
+        ("submit_s", C.c_double),
+        ("admit_s", C.c_double),
+        ("done_s", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_longlong),
+        ("completed", C.c_longlong),
+        ("chunks", C.c_longlong),
+        ("steps_cells", C.c_longlong),
+        ("bytes_in_use", C.c_size_t),
+        ("bytes_peak", C.c_size_t),
+        ("queued", C.c_int),
+        ("active", C.c_int),
+        ("ready", C.c_int),
+        ("device_ms", C.c_double),
     ]
 
 
@@ -152,6 +222,27 @@ def load_library(path: str | None = None):
     sig("lbm_group_set_launch", I, P, I, I)
     sig("lbm_group_get_launch", I, P, IP, IP)
     sig("lbm_group_autotune", I, P, I, IP, IP, C.POINTER(D))
+    sig("lbm_solver_required_bytes", I, C.POINTER(SolverConfig), C.POINTER(C.c_size_t))
+    sig("lbm_solver_create_in", I, C.POINTER(SolverConfig), P, C.c_size_t, P, C.POINTER(P))
+    sig("lbm_group_create_empty", I, I, I, I, I, C.POINTER(P))
+    sig("lbm_group_add_member", I, P, P, IP)
+    sig("lbm_group_remove_member", I, P, I)
+    sig("lbm_group_member_count", I, P, IP)
+    sig("lbm_group_set_member_stop", I, P, I, C.c_longlong)
+    sig("lbm_group_get_stream", I, P, C.POINTER(P))
+    sig("lbm_server_config_default", None, C.POINTER(ServerConfig))
+    sig("lbm_case_default", None, C.POINTER(Case))
+    sig("lbm_server_create", I, C.POINTER(ServerConfig), C.POINTER(P))
+    sig("lbm_server_destroy", I, P)
+    sig("lbm_server_error", C.c_char_p, P)
+    sig("lbm_server_case_bytes", I, C.POINTER(Case), C.POINTER(C.c_size_t))
+    sig("lbm_server_submit", I, P, C.POINTER(Case), C.POINTER(C.c_longlong))
+    sig("lbm_server_progress", I, P, IP)
+    sig("lbm_server_next_done", I, P, C.POINTER(CaseStatus))
+    sig("lbm_server_status", I, P, C.c_longlong, C.POINTER(CaseStatus))
+    sig("lbm_server_fetch", I, P, C.c_longlong, P, P)
+    sig("lbm_server_stats_get", I, P, C.POINTER(ServerStats))
+    sig("lbm_server_time", D, P)
     sig("lbm_group_set_timing", I, P, I)
     sig("lbm_group_get_timing", I, P, C.POINTER(D), C.POINTER(C.c_longlong), C.POINTER(D),
         C.POINTER(C.c_longlong))

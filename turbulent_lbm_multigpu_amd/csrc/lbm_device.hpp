@@ -182,6 +182,7 @@ struct DomDev {
     int store_velocity;
     int store_density;
     double inv_tau, tau0, les_coeff, gx, gy, gz, lid;
+    long long stop_at;  // launch half-step index from which this domain stays frozen
 };
 
 // ---- collision arithmetic (same operation order as the oracle) -------------
@@ -539,13 +540,23 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
 
 template <int ST, bool BETA, int VAR = 0>
 __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ doms, const int *__restrict__ tile_dom,
-                                                    int xcd_map) {
+                                                    int xcd_map, long long *ctr) {
     using SP = Storage<ST>;
     constexpr bool NTS = (VAR & kVarNtStore) != 0;
     constexpr int N = cells_per_lane(VAR);
+    // Half-step counter of a group launch: beta reads slot 0 and publishes
+    // h + 1 in slot 1 for the alpha launch after it, which does the reverse.
+    // Readers and the single writer of one launch use different slots, and
+    // the next launch on the stream sees the write.
+    long long h = 0;
+    if (ctr) {
+        h = ctr[BETA ? 0 : 1];
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctr[BETA ? 1 : 0] = h + 1;
+    }
     const int nb = (int)gridDim.x;
     const int t = xcd_map ? xcd_tile((int)blockIdx.x, nb) : (int)blockIdx.x;
     const DomDev *D = doms + tile_dom[t];
+    if (h >= D->stop_at) return;  // a finished member of a continuous batch keeps its state
     const int cells = (int)D->cells;
     const int block_cells = (int)blockDim.x * N;
     const int block_first = (t - D->tile_begin) * block_cells;

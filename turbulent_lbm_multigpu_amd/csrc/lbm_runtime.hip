@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -173,13 +174,18 @@ struct CopyOp {
     unsigned char pops[20];
     int block_begin;
     int cells;
+    long long stop_at;  // half-step index from which the receiving domain is frozen
 };
 
+// ctr: the group's half-step counter slot that the kernel just run has
+// written (h + 1), or null outside a counted group step.
 template <unsigned B>
-__global__ void __launch_bounds__(256) exchange_kernel(const CopyOp *__restrict__ ops, int nops) {
+__global__ void __launch_bounds__(256) exchange_kernel(const CopyOp *__restrict__ ops, int nops,
+                                                       const long long *ctr) {
     int k = 0;
     while (k + 1 < nops && ops[k + 1].block_begin <= (int)blockIdx.x) ++k;
     const CopyOp &op = ops[k];
+    if (ctr && *ctr - 1 >= op.stop_at) return;
     const int t = ((int)blockIdx.x - op.block_begin) * (int)blockDim.x + (int)threadIdx.x;
     if (t >= op.cells * op.npop) return;
     const int p = t / op.cells;
@@ -272,6 +278,11 @@ __global__ void __launch_bounds__(256) mass_energy_kernel(DomDev D, int swapped,
     }
 }
 
+__global__ void fill_int_kernel(int *p, int n, int v) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = v;
+}
+
 // Second pass: one block sums the partials in a fixed order (deterministic).
 __global__ void __launch_bounds__(256) sum_pairs_kernel(const double *partials, int n, double *out) {
     __shared__ double sm[2][256];
@@ -339,6 +350,8 @@ struct lbm_solver {
     size_t partial_count = 0;
     unsigned version = 0;      // bumps when kernel parameters change
     lbm_group *group = nullptr;
+    bool external = false;     // buffers live in caller-provided device memory
+    long long stop_at = std::numeric_limits<long long>::max();  // group half-step index of its freeze
 };
 
 struct GroupComm {
@@ -354,20 +367,26 @@ struct lbm_group {
     int variant = 0, lds = 0;  // launch shape of the fused step kernel (tuner-selected)
     std::string err;
     DomDev *d_descs = nullptr;
+    size_t descs_cap = 0;
     int *d_tiles = nullptr;
+    size_t tiles_cap = 0;
     int ntiles = 0;
+    long long *d_ctr = nullptr;  // device half-step counter (two slots, see step_kernel)
     std::vector<unsigned> seen_versions;
     std::vector<GroupComm> comms;
     bool plan_built = false;
+    bool descs_dirty = false;
     int exchange_mode = 0;
     // [0]=after alpha, [1]=after beta; per phase x/y/z
     CopyOp *d_ops[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    size_t ops_cap[2][3] = {{0, 0, 0}, {0, 0, 0}};
     int nops[2][3] = {{0, 0, 0}, {0, 0, 0}};
     int nblocks[2][3] = {{0, 0, 0}, {0, 0, 0}};
     long long step_counter = 0;
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
+    std::vector<int> graph_shape;  // launch sizes baked into the captured graph
     bool timing = false;
     struct Ev {
         hipEvent_t a, b;
@@ -428,6 +447,7 @@ void fill_desc(lbm_solver *s) {
     d.gy = s->cfg.gravitation[1];
     d.gz = s->cfg.gravitation[2];
     d.lid = s->cfg.lid_velocity;
+    d.stop_at = s->group ? s->stop_at : std::numeric_limits<long long>::max();
 }
 
 int upload_desc(lbm_solver *s) {
@@ -442,7 +462,7 @@ int upload_desc(lbm_solver *s) {
 // template arguments; bit 2 of the variant turns the XCD-contiguous tile map
 // off; lds_per_block reserves LDS to cap the number of resident work-groups.
 hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
-                              const int *tile_table, hipStream_t stream) {
+                              const int *tile_table, hipStream_t stream, long long *ctr = nullptr) {
     const int xcd = (variant & 4) ? 0 : 1;
     const int v = variant & ~4;
     dispatch_storage(storage, [&](auto st) -> int {
@@ -450,7 +470,7 @@ hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int 
         auto go = [&](auto b, auto vv) {
             constexpr bool B = decltype(b)::value;
             constexpr int V = decltype(vv)::value;
-            step_kernel<ST, B, V><<<tiles, wg, lds, stream>>>(descs, tile_table, xcd);
+            step_kernel<ST, B, V><<<tiles, wg, lds, stream>>>(descs, tile_table, xcd, ctr);
         };
         auto pick = [&](auto b) {
             switch (v) {
@@ -476,6 +496,10 @@ int launch_step(lbm_solver *s, bool beta) {
 
 int ensure_staging(lbm_solver *s, size_t bytes) {
     if (s->staging_bytes >= bytes) return LBM_OK;
+    if (s->external)
+        return fail(s, LBM_ERR_CAPACITY,
+                    "a solver in caller-provided memory has " + std::to_string(s->staging_bytes) +
+                        " staging bytes; this transfer needs " + std::to_string(bytes) + ": transfer it in regions");
     if (s->staging) SCHK(hipFree(s->staging));
     s->staging = nullptr;
     s->staging_bytes = 0;
@@ -531,10 +555,48 @@ extern "C" void lbm_solver_config_default(lbm_solver_config *c) {
 
 // ---- solver ------------------------------------------------------------------
 
-extern "C" int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out) {
-    if (!cfg || !out) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "lbm_solver_create: null argument");
-    *out = nullptr;
-    const lbm_solver_config &c = *cfg;
+// Device-memory layout of one solver: every buffer 256-byte aligned, in the
+// order distributions, flags, velocity, density, launch descriptor, tile
+// table, staging.  Staging holds one byte per cell (obstacle masks) or the 19
+// populations of the largest face, whichever is more; larger transfers go
+// region by region.
+struct SolverLayout {
+    size_t dd = 0, flags = 0, vel = 0, dens = 0, desc = 0, tile = 0, staging = 0, staging_bytes = 0, total = 0;
+};
+
+static SolverLayout solver_layout(const lbm_solver_config &c, long long cells, long long plane, int ntiles) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t es = elem_size(c.storage), cs = compute_size(c.storage);
+    SolverLayout L;
+    size_t off = 0;
+    L.dd = off;
+    off += al((size_t)19 * plane * es);
+    L.flags = off;
+    off += al((size_t)cells);
+    if (c.store_velocity) {
+        L.vel = off;
+        off += al((size_t)3 * cells * cs);
+    }
+    if (c.store_density) {
+        L.dens = off;
+        off += al((size_t)cells * cs);
+    }
+    L.desc = off;
+    off += al(sizeof(DomDev));
+    L.tile = off;
+    off += al(sizeof(int) * (size_t)ntiles);
+    L.staging = off;
+    {
+        const size_t X = c.dims[0], Y = c.dims[1], Z = c.dims[2];
+        const size_t face = std::max(X * Y, std::max(Y * Z, X * Z));
+        L.staging_bytes = std::max((size_t)cells, (size_t)19 * cs * face);
+    }
+    off += al(L.staging_bytes);
+    L.total = off;
+    return L;
+}
+
+static int validate_config(const lbm_solver_config &c) {
     for (int a = 0; a < 3; ++a)
         if (c.dims[a] < 3)
             return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "dims must be >= 3 on every axis");
@@ -550,55 +612,95 @@ extern "C" int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out)
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
                       "one distribution plane must stay below 4 GiB: split the domain into subdomains");
     if (c.forcing != 0 && c.forcing != 1) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "unknown forcing");
-    int wg = c.work_group_size == 0 ? 256 : c.work_group_size;
+    const int wg = c.work_group_size == 0 ? 256 : c.work_group_size;
     if (wg < 64 || wg > 256 || wg % 64 != 0)
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "work_group_size must be 64, 128, 192 or 256");
     if (!(c.inv_tau > 0.0) || !std::isfinite(c.inv_tau))
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "inv_tau must be positive and finite");
     if (c.smagorinsky_cs < 0.0 || (c.smagorinsky_cs > 0.0 && !(c.tau > 0.5)))
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "LES needs smagorinsky_cs >= 0 and tau > 0.5");
-    {
-        const int v = c.kernel_variant & ~4;
-        if (c.kernel_variant < 0 || (v != 0 && v != 2 && v != 8 && v != 16) || c.lds_per_block < 0 ||
-            c.lds_per_block > 65536)
-            return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
-                          "kernel_variant must be one of 0, 2, 8, 16 (optionally | 4); lds_per_block in [0, 65536]");
-    }
+    const int v = c.kernel_variant & ~4;
+    if (c.kernel_variant < 0 || (v != 0 && v != 2 && v != 8 && v != 16) || c.lds_per_block < 0 ||
+        c.lds_per_block > 65536)
+        return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
+                      "kernel_variant must be one of 0, 2, 8, 16 (optionally | 4); lds_per_block in [0, 65536]");
+    return LBM_OK;
+}
+
+static void solver_geometry(const lbm_solver_config &c, long long &cells, long long &plane, int &wg, int &ntiles) {
+    cells = (long long)c.dims[0] * c.dims[1] * c.dims[2];
+    const long long pad = c.plane_padding < 0 ? 0 : c.plane_padding;
+    plane = ((cells + 255) / 256) * 256 + pad;
+    wg = c.work_group_size == 0 ? 256 : c.work_group_size;
+    const long long per_tile = (long long)wg * cells_per_lane(c.kernel_variant);
+    ntiles = (int)((cells + per_tile - 1) / per_tile);
+}
+
+static int create_solver(const lbm_solver_config *cfg, void *memory, size_t memory_bytes, hipStream_t stream,
+                         lbm_solver **out) {
+    if (!cfg || !out) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "lbm_solver_create: null argument");
+    *out = nullptr;
+    const lbm_solver_config &c = *cfg;
+    int rc0 = validate_config(c);
+    if (rc0) return rc0;
+    long long cells, plane;
+    int wg, ntiles;
+    solver_geometry(c, cells, plane, wg, ntiles);
+    const SolverLayout L = solver_layout(c, cells, plane, ntiles);
+    if (memory && (memory_bytes < L.total || ((uintptr_t)memory & 255) != 0))
+        return record(nullptr, LBM_ERR_CAPACITY,
+                      "device memory block of " + std::to_string(memory_bytes) + " bytes (256-byte aligned) is smaller than the " +
+                          std::to_string(L.total) + " bytes this solver needs");
 
     lbm_solver *s = new lbm_solver();
     s->cfg = c;
     s->cfg.work_group_size = wg;
     s->wg = wg;
     s->cells = cells;
-    const long long pad = c.plane_padding < 0 ? 0 : c.plane_padding;
-    s->plane = ((cells + 255) / 256) * 256 + pad;
-    {
-        const long long per_tile = (long long)wg * cells_per_lane(c.kernel_variant);
-        s->ntiles = (int)((cells + per_tile - 1) / per_tile);
-    }
+    s->plane = plane;
+    s->ntiles = ntiles;
+    s->external = memory != nullptr;
     auto bail = [&](int code, const std::string &m) {
         record(nullptr, code, m);
         lbm_solver_destroy(s);
         return code;
     };
     if (hipSetDevice(c.device) != hipSuccess) return bail(LBM_ERR_DEVICE, "hipSetDevice failed");
-    if (hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(LBM_ERR_DEVICE, "hipStreamCreate failed");
-    s->stream = s->own_stream;
-    const size_t es = elem_size(c.storage), cs = compute_size(c.storage);
-    if (hipMalloc(&s->dd, (size_t)19 * s->plane * es) != hipSuccess)
-        return bail(LBM_ERR_OUT_OF_MEMORY, "distribution buffer allocation failed");
-    if (hipMalloc((void **)&s->flags, (size_t)cells) != hipSuccess)
-        return bail(LBM_ERR_OUT_OF_MEMORY, "flag buffer allocation failed");
-    if (c.store_velocity && hipMalloc(&s->vel, (size_t)3 * cells * cs) != hipSuccess)
-        return bail(LBM_ERR_OUT_OF_MEMORY, "velocity buffer allocation failed");
-    if (c.store_density && hipMalloc(&s->dens, (size_t)cells * cs) != hipSuccess)
-        return bail(LBM_ERR_OUT_OF_MEMORY, "density buffer allocation failed");
-    if (hipMalloc((void **)&s->d_desc, sizeof(DomDev)) != hipSuccess ||
-        hipMalloc((void **)&s->d_tile, sizeof(int) * (size_t)s->ntiles) != hipSuccess)
-        return bail(LBM_ERR_OUT_OF_MEMORY, "descriptor allocation failed");
-    if (hipMemsetAsync(s->d_tile, 0, sizeof(int) * (size_t)s->ntiles, s->stream) != hipSuccess)
-        return bail(LBM_ERR_DEVICE, "tile table initialisation failed");
+    if (stream) {
+        s->stream = stream;
+    } else {
+        if (hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess)
+            return bail(LBM_ERR_DEVICE, "hipStreamCreate failed");
+        s->stream = s->own_stream;
+    }
+    if (memory) {
+        char *base = (char *)memory;
+        s->dd = base + L.dd;
+        s->flags = (uint8_t *)(base + L.flags);
+        s->vel = c.store_velocity ? base + L.vel : nullptr;
+        s->dens = c.store_density ? base + L.dens : nullptr;
+        s->d_desc = (DomDev *)(base + L.desc);
+        s->d_tile = (int *)(base + L.tile);
+        s->staging = base + L.staging;
+        s->staging_bytes = L.staging_bytes;
+    } else {
+        const size_t es = elem_size(c.storage), cs = compute_size(c.storage);
+        if (hipMalloc(&s->dd, (size_t)19 * s->plane * es) != hipSuccess)
+            return bail(LBM_ERR_OUT_OF_MEMORY, "distribution buffer allocation failed");
+        if (hipMalloc((void **)&s->flags, (size_t)cells) != hipSuccess)
+            return bail(LBM_ERR_OUT_OF_MEMORY, "flag buffer allocation failed");
+        if (c.store_velocity && hipMalloc(&s->vel, (size_t)3 * cells * cs) != hipSuccess)
+            return bail(LBM_ERR_OUT_OF_MEMORY, "velocity buffer allocation failed");
+        if (c.store_density && hipMalloc(&s->dens, (size_t)cells * cs) != hipSuccess)
+            return bail(LBM_ERR_OUT_OF_MEMORY, "density buffer allocation failed");
+        if (hipMalloc((void **)&s->d_desc, sizeof(DomDev)) != hipSuccess ||
+            hipMalloc((void **)&s->d_tile, sizeof(int) * (size_t)s->ntiles) != hipSuccess)
+            return bail(LBM_ERR_OUT_OF_MEMORY, "descriptor allocation failed");
+    }
+    // all tiles of the single-domain launch belong to domain 0 (a fill
+    // kernel, not a memset: callers may capture later launches in graphs)
+    fill_int_kernel<<<(unsigned)((s->ntiles + 255) / 256), 256, 0, s->stream>>>(s->d_tile, s->ntiles, 0);
+    if (hipGetLastError() != hipSuccess) return bail(LBM_ERR_DEVICE, "tile table initialisation failed");
     if (upload_desc(s) != LBM_OK) return bail(LBM_ERR_DEVICE, s->err);
     int rc = lbm_solver_reset(s);
     if (rc != LBM_OK) return bail(rc, s->err);
@@ -606,18 +708,41 @@ extern "C" int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out)
     return LBM_OK;
 }
 
+extern "C" int lbm_solver_create(const lbm_solver_config *cfg, lbm_solver **out) {
+    return create_solver(cfg, nullptr, 0, nullptr, out);
+}
+
+extern "C" int lbm_solver_required_bytes(const lbm_solver_config *cfg, size_t *bytes) {
+    if (!cfg || !bytes) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = validate_config(*cfg);
+    if (rc) return rc;
+    long long cells, plane;
+    int wg, ntiles;
+    solver_geometry(*cfg, cells, plane, wg, ntiles);
+    *bytes = solver_layout(*cfg, cells, plane, ntiles).total;
+    return LBM_OK;
+}
+
+extern "C" int lbm_solver_create_in(const lbm_solver_config *cfg, void *device_memory, size_t bytes, void *stream,
+                                    lbm_solver **out) {
+    if (!device_memory) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "lbm_solver_create_in: null device memory");
+    return create_solver(cfg, device_memory, bytes, (hipStream_t)stream, out);
+}
+
 extern "C" int lbm_solver_destroy(lbm_solver *s) {
     if (!s) return LBM_OK;
     if (s->group) return fail(s, LBM_ERR_STATE, "destroy the group before its member solvers");
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    (void)hipFree(s->dd);
-    (void)hipFree(s->flags);
-    (void)hipFree(s->vel);
-    (void)hipFree(s->dens);
-    (void)hipFree(s->d_desc);
-    (void)hipFree(s->d_tile);
-    (void)hipFree(s->staging);
-    (void)hipFree(s->d_partials);
+    if (!s->external) {
+        (void)hipFree(s->dd);
+        (void)hipFree(s->flags);
+        (void)hipFree(s->vel);
+        (void)hipFree(s->dens);
+        (void)hipFree(s->d_desc);
+        (void)hipFree(s->d_tile);
+        (void)hipFree(s->staging);
+    }
+    if (s->d_partials) (void)hipFree(s->d_partials);
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
     delete s;
     return LBM_OK;
@@ -902,14 +1027,16 @@ static int fill_op(lbm_solver *src, const int *so, lbm_solver *dst, const int *d
     }
     op.cells = (int)cells;
     op.block_begin = 0;
+    op.stop_at = dst->stop_at;
     return LBM_OK;
 }
 
-static hipError_t exchange_launch(int storage, const CopyOp *d_ops, int nops, int nblocks, hipStream_t stream) {
+static hipError_t exchange_launch(int storage, const CopyOp *d_ops, int nops, int nblocks, hipStream_t stream,
+                                  const long long *ctr = nullptr) {
     switch (elem_size(storage)) {
-        case 8: exchange_kernel<8><<<nblocks, 256, 0, stream>>>(d_ops, nops); break;
-        case 4: exchange_kernel<4><<<nblocks, 256, 0, stream>>>(d_ops, nops); break;
-        default: exchange_kernel<2><<<nblocks, 256, 0, stream>>>(d_ops, nops); break;
+        case 8: exchange_kernel<8><<<nblocks, 256, 0, stream>>>(d_ops, nops, ctr); break;
+        case 4: exchange_kernel<4><<<nblocks, 256, 0, stream>>>(d_ops, nops, ctr); break;
+        default: exchange_kernel<2><<<nblocks, 256, 0, stream>>>(d_ops, nops, ctr); break;
     }
     return hipGetLastError();
 }
@@ -1099,23 +1226,47 @@ extern "C" int lbm_device_copy_bandwidth(int device, size_t bytes, int reps, int
 
 // ---- group -----------------------------------------------------------------
 
+static void group_drop_graph(lbm_group *g) {
+    if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->graph_exec = nullptr;
+    g->graph = nullptr;
+    g->graph_shape.clear();
+}
+
 static void group_free_plan(lbm_group *g) {
     for (int k = 0; k < 2; ++k)
         for (int p = 0; p < 3; ++p) {
             if (g->d_ops[k][p]) (void)hipFree(g->d_ops[k][p]);
             g->d_ops[k][p] = nullptr;
+            g->ops_cap[k][p] = 0;
             g->nops[k][p] = 0;
             g->nblocks[k][p] = 0;
         }
-    if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
-    if (g->graph) (void)hipGraphDestroy(g->graph);
-    g->graph_exec = nullptr;
-    g->graph = nullptr;
+    group_drop_graph(g);
     g->plan_built = false;
 }
 
+// Device tables grow by doubling and are never shrunk, so membership changes
+// of a continuous batch rewrite them in place (no free, no device-wide wait).
+template <typename T>
+static hipError_t grow_table(T *&ptr, size_t &cap, size_t n) {
+    if (ptr && n <= cap) return hipSuccess;
+    size_t c = std::max<size_t>(std::max<size_t>(n, 2 * cap), 16);
+    if (ptr) {
+        hipError_t e = hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipMalloc((void **)&ptr, sizeof(T) * c);
+    if (e == hipSuccess) cap = c;
+    return e;
+}
+
 // Descriptor and tile tables of the fused launch for the group's current
-// launch shape (work-group size and cells per lane).
+// members and launch shape (work-group size and cells per lane); also
+// re-publishes the host half-step count to the device counter.
 static int group_upload_descs(lbm_group *g) {
     std::vector<DomDev> descs;
     std::vector<int> tiles;
@@ -1130,17 +1281,26 @@ static int group_upload_descs(lbm_group *g) {
         descs.push_back(d);
         tiles.insert(tiles.end(), (size_t)nt, (int)i);
     }
-    if ((int)tiles.size() != g->ntiles || !g->d_tiles) {
-        if (g->d_tiles) GCHK(hipFree(g->d_tiles));
-        g->d_tiles = nullptr;
-        GCHK(hipMalloc((void **)&g->d_tiles, sizeof(int) * tiles.size()));
-        g->ntiles = (int)tiles.size();
-    }
-    GCHK(hipMemcpyAsync(g->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, g->stream));
-    GCHK(hipMemcpyAsync(g->d_descs, descs.data(), sizeof(DomDev) * descs.size(), hipMemcpyHostToDevice, g->stream));
+    GCHK(grow_table(g->d_tiles, g->tiles_cap, tiles.size()));
+    GCHK(grow_table(g->d_descs, g->descs_cap, descs.size()));
+    g->ntiles = (int)tiles.size();
+    if (!tiles.empty())
+        GCHK(hipMemcpyAsync(g->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, g->stream));
+    if (!descs.empty())
+        GCHK(hipMemcpyAsync(g->d_descs, descs.data(), sizeof(DomDev) * descs.size(), hipMemcpyHostToDevice, g->stream));
+    const long long ctr[2] = {g->step_counter, g->step_counter};
+    GCHK(hipMemcpyAsync(g->d_ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, g->stream));
     GCHK(hipStreamSynchronize(g->stream));
     for (size_t i = 0; i < g->members.size(); ++i) g->seen_versions[i] = g->members[i]->version;
+    g->descs_dirty = false;
     return LBM_OK;
+}
+
+static int group_init(lbm_group *g) {
+    GCHK(hipSetDevice(g->device));
+    GCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    GCHK(hipMalloc((void **)&g->d_ctr, 2 * sizeof(long long)));
+    return group_upload_descs(g);
 }
 
 extern "C" int lbm_group_create(lbm_solver *const *solvers, int n, lbm_group **out) {
@@ -1157,6 +1317,10 @@ extern "C" int lbm_group_create(lbm_solver *const *solvers, int n, lbm_group **o
                           "group members need the same storage, work-group size, kernel variant and device");
         for (int j = 0; j < i; ++j)
             if (solvers[j] == solvers[i]) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "duplicate member solver");
+        // members advance together, so their step counters must share a parity
+        // (the AA pattern's layout); each keeps its own count
+        if ((solvers[i]->step_counter & 1) != (solvers[0]->step_counter & 1))
+            return record(nullptr, LBM_ERR_STATE, "member step counters differ in parity");
     }
     lbm_group *g = new lbm_group();
     g->storage = solvers[0]->cfg.storage;
@@ -1164,32 +1328,53 @@ extern "C" int lbm_group_create(lbm_solver *const *solvers, int n, lbm_group **o
     g->device = solvers[0]->cfg.device;
     g->variant = solvers[0]->cfg.kernel_variant;
     g->lds = solvers[0]->cfg.lds_per_block;
-    for (int i = 0; i < n; ++i) g->members.push_back(solvers[i]);
-    g->seen_versions.assign(n, ~0u);
-    auto bail = [&](int code, const std::string &m) {
-        record(nullptr, code, m);
-        for (auto *mm : g->members) mm->group = nullptr;
-        lbm_group_destroy(g);
-        return code;
-    };
-    if (hipSetDevice(g->device) != hipSuccess) return bail(LBM_ERR_DEVICE, "hipSetDevice failed");
-    for (auto *m : g->members)
-        if (hipStreamSynchronize(m->stream) != hipSuccess) return bail(LBM_ERR_DEVICE, "member stream sync failed");
-    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(LBM_ERR_DEVICE, "hipStreamCreate failed");
-    if (hipMalloc((void **)&g->d_descs, sizeof(DomDev) * n) != hipSuccess)
-        return bail(LBM_ERR_OUT_OF_MEMORY, "group table allocation failed");
-    // members advance together, so their step counters must share a parity
-    // (the AA pattern's layout); each keeps its own count
     g->step_counter = solvers[0]->step_counter;
-    for (auto *m : g->members) {
-        if ((m->step_counter & 1) != (g->step_counter & 1))
-            return bail(LBM_ERR_STATE, "member step counters differ in parity");
-        g->saved_streams.push_back(m->stream);
-        m->stream = g->stream;
-        m->group = g;
+    if (hipSetDevice(g->device) != hipSuccess) {
+        delete g;
+        return record(nullptr, LBM_ERR_DEVICE, "hipSetDevice failed");
     }
-    if (group_upload_descs(g) != LBM_OK) return bail(LBM_ERR_DEVICE, g->err);
+    for (int i = 0; i < n; ++i) {
+        lbm_solver *m = solvers[i];
+        if (hipStreamSynchronize(m->stream) != hipSuccess) {
+            lbm_group_destroy(g);
+            return record(nullptr, LBM_ERR_DEVICE, "member stream sync failed");
+        }
+        g->members.push_back(m);
+        g->saved_streams.push_back(m->stream);
+        g->seen_versions.push_back(~0u);
+        m->group = g;
+        m->stop_at = std::numeric_limits<long long>::max();
+    }
+    int rc = group_init(g);
+    if (rc) {
+        std::string msg = g->err;
+        lbm_group_destroy(g);
+        return record(nullptr, rc, msg);
+    }
+    for (auto *m : g->members) m->stream = g->stream;
+    *out = g;
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_create_empty(int device, int storage, int work_group_size, int kernel_variant,
+                                      lbm_group **out) {
+    if (!out) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    const int v = kernel_variant & ~4;
+    if (storage < 0 || storage > 3 || work_group_size < 64 || work_group_size > 256 || work_group_size % 64 != 0 ||
+        kernel_variant < 0 || (v != 0 && v != 2 && v != 8 && v != 16))
+        return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad storage or launch shape");
+    lbm_group *g = new lbm_group();
+    g->storage = storage;
+    g->wg = work_group_size;
+    g->device = device;
+    g->variant = kernel_variant;
+    int rc = group_init(g);
+    if (rc) {
+        std::string msg = g->err;
+        lbm_group_destroy(g);
+        return record(nullptr, rc, msg);
+    }
     *out = g;
     return LBM_OK;
 }
@@ -1202,18 +1387,90 @@ extern "C" int lbm_group_destroy(lbm_group *g) {
         (void)hipEventDestroy(e.b);
     }
     group_free_plan(g);
-    for (size_t i = 0; i < g->members.size() && i < g->saved_streams.size(); ++i) {
-        g->members[i]->stream = g->saved_streams[i];
-        g->members[i]->group = nullptr;
+    for (size_t i = 0; i < g->members.size(); ++i) {
+        lbm_solver *m = g->members[i];
+        if (i < g->saved_streams.size()) m->stream = g->saved_streams[i];
+        m->group = nullptr;
+        m->stop_at = std::numeric_limits<long long>::max();
     }
     (void)hipFree(g->d_descs);
     (void)hipFree(g->d_tiles);
+    (void)hipFree(g->d_ctr);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return LBM_OK;
 }
 
 extern "C" const char *lbm_group_error(const lbm_group *g) { return g ? g->err.c_str() : g_last_error.c_str(); }
+
+extern "C" int lbm_group_add_member(lbm_group *g, lbm_solver *s, int *index) {
+    if (!g || !s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->group) return gfail(g, LBM_ERR_STATE, "a solver can belong to one group only");
+    if (s->cfg.storage != g->storage || s->cfg.device != g->device)
+        return gfail(g, LBM_ERR_INVALID_ARGUMENT, "a member needs the group's storage and device");
+    if ((s->step_counter & 1) != (g->step_counter & 1))
+        return gfail(g, LBM_ERR_STATE, "the solver's step counter differs in parity from the group's");
+    GCHK(hipSetDevice(g->device));
+    GCHK(hipStreamSynchronize(s->stream));
+    GCHK(hipStreamSynchronize(g->stream));
+    g->members.push_back(s);
+    g->saved_streams.push_back(s->stream);
+    g->seen_versions.push_back(~0u);
+    s->group = g;
+    s->stop_at = std::numeric_limits<long long>::max();
+    s->stream = g->stream;
+    g->descs_dirty = true;
+    g->plan_built = false;
+    if (index) *index = (int)g->members.size() - 1;
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_remove_member(lbm_group *g, int index) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    if (index < 0 || index >= (int)g->members.size()) return gfail(g, LBM_ERR_INVALID_ARGUMENT, "member index out of range");
+    GCHK(hipStreamSynchronize(g->stream));
+    lbm_solver *m = g->members[index];
+    m->stream = g->saved_streams[index];
+    m->group = nullptr;
+    m->stop_at = std::numeric_limits<long long>::max();
+    g->members.erase(g->members.begin() + index);
+    g->saved_streams.erase(g->saved_streams.begin() + index);
+    g->seen_versions.erase(g->seen_versions.begin() + index);
+    std::vector<GroupComm> kept;
+    for (GroupComm c : g->comms) {
+        if (c.member == index || c.dst == index) continue;
+        if (c.member > index) --c.member;
+        if (c.dst > index) --c.dst;
+        kept.push_back(c);
+    }
+    g->comms.swap(kept);
+    g->descs_dirty = true;
+    g->plan_built = false;
+    if (upload_desc(m) != LBM_OK) return gfail(g, LBM_ERR_DEVICE, m->err);
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_get_stream(const lbm_group *g, void **stream) {
+    if (!g || !stream) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    *stream = (void *)g->stream;
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_member_count(const lbm_group *g, int *count) {
+    if (!g || !count) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    *count = (int)g->members.size();
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_set_member_stop(lbm_group *g, int index, long long steps) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    if (index < 0 || index >= (int)g->members.size()) return gfail(g, LBM_ERR_INVALID_ARGUMENT, "member index out of range");
+    lbm_solver *m = g->members[index];
+    m->stop_at = steps < 0 ? std::numeric_limits<long long>::max() : g->step_counter + steps;
+    m->version++;
+    g->plan_built = false;  // the exchange copies carry the receiver's stop
+    return LBM_OK;
+}
 
 extern "C" int lbm_group_add_comm(lbm_group *g, int member, int dst_member, const int send_origin[3],
                                   const int recv_origin[3], const int size[3], const int normal[3]) {
@@ -1236,14 +1493,14 @@ extern "C" int lbm_group_add_comm(lbm_group *g, int member, int dst_member, cons
         c.normal[a] = normal[a];
     }
     g->comms.push_back(c);
-    group_free_plan(g);
+    g->plan_built = false;
     return LBM_OK;
 }
 
 extern "C" int lbm_group_set_exchange_mode(lbm_group *g, int mode) {
     if (!g || (mode != 0 && mode != 1)) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "exchange mode must be 0 or 1");
     g->exchange_mode = mode;
-    group_free_plan(g);
+    g->plan_built = false;
     return LBM_OK;
 }
 
@@ -1297,14 +1554,42 @@ static int build_plan(lbm_group *g) {
             g->nops[k][p] = (int)ops[k][p].size();
             g->nblocks[k][p] = blocks;
             if (ops[k][p].empty()) continue;
-            GCHK(hipMalloc((void **)&g->d_ops[k][p], sizeof(CopyOp) * ops[k][p].size()));
-            GCHK(hipMemcpy(g->d_ops[k][p], ops[k][p].data(), sizeof(CopyOp) * ops[k][p].size(), hipMemcpyHostToDevice));
+            GCHK(grow_table(g->d_ops[k][p], g->ops_cap[k][p], ops[k][p].size()));
+            GCHK(hipMemcpyAsync(g->d_ops[k][p], ops[k][p].data(), sizeof(CopyOp) * ops[k][p].size(),
+                                hipMemcpyHostToDevice, g->stream));
         }
+    GCHK(hipStreamSynchronize(g->stream));
     g->plan_built = true;
     return LBM_OK;
 }
 
+// Brings descriptors and the exchange plan up to date before launches.
+static int group_prepare(lbm_group *g) {
+    bool stale = g->descs_dirty;
+    for (size_t i = 0; i < g->members.size(); ++i) stale |= g->seen_versions[i] != g->members[i]->version;
+    if (stale) {
+        int rc = group_upload_descs(g);
+        if (rc) return rc;
+    }
+    if (!g->plan_built) {
+        int rc = build_plan(g);
+        if (rc) return rc;
+    }
+    return LBM_OK;
+}
+
+static std::vector<int> group_shape(const lbm_group *g) {
+    std::vector<int> v = {g->ntiles, g->wg, g->variant, g->lds};
+    for (int k = 0; k < 2; ++k)
+        for (int p = 0; p < 3; ++p) {
+            v.push_back(g->nops[k][p]);
+            v.push_back(g->nblocks[k][p]);
+        }
+    return v;
+}
+
 static int group_launch_half(lbm_group *g, bool beta, bool timed) {
+    if (g->ntiles == 0) return LBM_OK;
     hipEvent_t ea = nullptr, eb = nullptr;
     if (timed) {
         GCHK(hipEventCreate(&ea));
@@ -1312,7 +1597,7 @@ static int group_launch_half(lbm_group *g, bool beta, bool timed) {
         GCHK(hipEventRecord(ea, g->stream));
     }
     GCHK(launch_step_kernel(g->storage, beta, g->variant, g->lds, g->ntiles, g->wg, g->d_descs, g->d_tiles,
-                            g->stream));
+                            g->stream, g->d_ctr));
     if (timed) {
         GCHK(hipEventRecord(eb, g->stream));
         g->events.push_back({ea, eb, beta ? 0 : 2});
@@ -1327,7 +1612,8 @@ static int group_launch_half(lbm_group *g, bool beta, bool timed) {
             GCHK(hipEventCreate(&xb));
             GCHK(hipEventRecord(xa, g->stream));
         }
-        GCHK(exchange_launch(g->storage, g->d_ops[k][p], g->nops[k][p], g->nblocks[k][p], g->stream));
+        GCHK(exchange_launch(g->storage, g->d_ops[k][p], g->nops[k][p], g->nblocks[k][p], g->stream,
+                             g->d_ctr + (beta ? 1 : 0)));
         if (timed) {
             GCHK(hipEventRecord(xb, g->stream));
             g->events.push_back({xa, xb, 1});
@@ -1336,23 +1622,33 @@ static int group_launch_half(lbm_group *g, bool beta, bool timed) {
     return LBM_OK;
 }
 
+// Advances every member's own step counter by the half-steps it took part in
+// (a member frozen at its stop keeps its count).
+static void group_advance_counters(lbm_group *g, long long before, long long done) {
+    for (auto *m : g->members) {
+        const long long room = m->stop_at - before;
+        m->step_counter += room <= 0 ? 0 : std::min(done, room);
+    }
+    g->step_counter = before + done;
+}
+
 extern "C" int lbm_group_step(lbm_group *g, int steps) {
     if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
     if (steps < 0) return gfail(g, LBM_ERR_INVALID_ARGUMENT, "steps must be >= 0");
     GCHK(hipSetDevice(g->device));
-    bool stale = false;
-    for (size_t i = 0; i < g->members.size(); ++i) stale |= g->seen_versions[i] != g->members[i]->version;
-    if (stale) {
-        int rc = group_upload_descs(g);
-        if (rc) return rc;
+    int rc = group_prepare(g);
+    if (rc) return rc;
+    const long long before = g->step_counter;
+    if (g->ntiles == 0) {  // no members: counters only (re-published with the next upload)
+        g->descs_dirty = true;
+        group_advance_counters(g, before, steps);
+        return LBM_OK;
     }
-    if (!g->plan_built) {
-        int rc = build_plan(g);
-        if (rc) return rc;
-    }
+    if (g->graph_exec && g->graph_shape != group_shape(g)) group_drop_graph(g);
+    long long h = before;
     int done = 0;
     while (done < steps) {
-        const bool beta = (g->step_counter & 1) == 0;
+        const bool beta = (h & 1) == 0;
         if (g->use_graph && !g->timing && beta && steps - done >= 2) {
             if (!g->graph_exec) {
                 GCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
@@ -1367,18 +1663,19 @@ extern "C" int lbm_group_step(lbm_group *g, int steps) {
                 GCHK(ec);
                 g->graph = graph;
                 GCHK(hipGraphInstantiate(&g->graph_exec, g->graph, nullptr, nullptr, 0));
+                g->graph_shape = group_shape(g);
             }
             GCHK(hipGraphLaunch(g->graph_exec, g->stream));
-            g->step_counter += 2;
+            h += 2;
             done += 2;
         } else {
-            int rc = group_launch_half(g, beta, g->timing);
+            rc = group_launch_half(g, beta, g->timing);
             if (rc) return rc;
-            g->step_counter += 1;
+            h += 1;
             done += 1;
         }
     }
-    for (auto *m : g->members) m->step_counter += done;
+    group_advance_counters(g, before, done);
     return LBM_OK;
 }
 
@@ -1391,10 +1688,7 @@ extern "C" int lbm_group_set_launch(lbm_group *g, int work_group_size, int kerne
     GCHK(hipStreamSynchronize(g->stream));
     g->wg = work_group_size;
     g->variant = kernel_variant;
-    if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
-    if (g->graph) (void)hipGraphDestroy(g->graph);
-    g->graph_exec = nullptr;
-    g->graph = nullptr;
+    group_drop_graph(g);
     return group_upload_descs(g);
 }
 
@@ -1418,10 +1712,11 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
         int rc = lbm_group_step(g, 1);  // start the pairs on a beta step
         if (rc) return rc;
     }
-    if (!g->plan_built) {
-        int rc = build_plan(g);
+    {
+        int rc = group_prepare(g);
         if (rc) return rc;
     }
+    if (g->ntiles == 0) return gfail(g, LBM_ERR_STATE, "autotune needs at least one member");
     const int wgs[3] = {64, 128, 256};
     const int vars[3] = {0, 8, 16};
     double best = 1e300;
@@ -1446,8 +1741,7 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
             GCHK(hipEventSynchronize(e1));
             float ms = 0.f;
             GCHK(hipEventElapsedTime(&ms, e0, e1));
-            g->step_counter += 2 * (pairs + 1);
-            for (auto *m : g->members) m->step_counter += 2 * (pairs + 1);
+            group_advance_counters(g, g->step_counter, 2 * (pairs + 1));
             const double per = (double)ms / (2.0 * pairs);
             if (per < best) {
                 best = per;
@@ -1468,8 +1762,8 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
 extern "C" int lbm_group_sync(lbm_group *g, int after_beta) {
     if (!g || (after_beta != 0 && after_beta != 1)) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad argument");
     GCHK(hipSetDevice(g->device));
-    if (!g->plan_built) {
-        int rc = build_plan(g);
+    {
+        int rc = group_prepare(g);
         if (rc) return rc;
     }
     for (int p = 0; p < 3; ++p) {

@@ -233,9 +233,48 @@ class LbmGroup:
         for s in self.solvers:
             s._group_owner = self
 
+    @classmethod
+    def empty(cls, storage: int = _capi.STORAGE_F32, work_group_size: int = 256, kernel_variant: int = 0,
+              device: int = 0) -> "LbmGroup":
+        """A group with no members yet (members join with add_member)."""
+        g = cls.__new__(cls)
+        g._lib = _capi.load_library()
+        h = C.c_void_p()
+        rc = g._lib.lbm_group_create_empty(int(device), int(storage), int(work_group_size), int(kernel_variant),
+                                           C.byref(h))
+        if rc != 0:
+            raise LbmError(rc, g._lib.lbm_last_error().decode())
+        g._h = h
+        g.solvers = []
+        return g
+
     def _check(self, rc):
         if rc != 0:
             raise LbmError(rc, self._lib.lbm_group_error(self._h).decode())
+
+    def add_member(self, solver: "LbmSolver") -> int:
+        """Join between steps (step counter of the group's parity); returns the member index."""
+        idx = C.c_int()
+        self._check(self._lib.lbm_group_add_member(self._h, solver.handle, C.byref(idx)))
+        self.solvers.append(solver)
+        solver._group_owner = self
+        return idx.value
+
+    def remove_member(self, index: int) -> "LbmSolver":
+        """Leave with the state in place; later members are renumbered."""
+        self._check(self._lib.lbm_group_remove_member(self._h, int(index)))
+        s = self.solvers.pop(int(index))
+        s._group_owner = None
+        return s
+
+    def set_member_stop(self, index: int, steps: int):
+        """Member `index` freezes after `steps` more half-steps (negative: never)."""
+        self._check(self._lib.lbm_group_set_member_stop(self._h, int(index), int(steps)))
+
+    def member_count(self) -> int:
+        n = C.c_int()
+        self._check(self._lib.lbm_group_member_count(self._h, C.byref(n)))
+        return n.value
 
     def add_comm(self, member: int, dst_member: int, send_origin, recv_origin, size, normal):
         self._check(self._lib.lbm_group_add_comm(self._h, int(member), int(dst_member), ivec(send_origin),
