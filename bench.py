@@ -9,8 +9,16 @@ the configured 256^3 cells per step (src/CController.hpp:446-452).
 
 Prints one JSON line (rank 0).  The roofline entry divides the algorithmic
 152 B per cell update (19 loads + 19 stores of FP32) by the measured average
-duration of the step kernel (events on the group's stream).  The CPU baseline
-times the C oracle (OpenMP) on a bounded number of steps of the same workload.
+duration of the step kernel (events on the group's stream); its traffic is the
+HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in
+profiles/ when that was taken with the same launch configuration.  The CPU
+baseline times the C oracle (OpenMP) on a bounded number of steps of the same
+workload.  Two more workloads are reported beside the headline number:
+batching (512 uneven LES cavities, 1000 steps, one fused launch per
+half-step, autotuned shape, against one case at a time) and serving (a
+seeded cavity/channel request stream through the continuous-batching server:
+FP16-shifted storage with batching against FP32 one case at a time, cases/s
+at saturation and p50/p99 latency at a Poisson arrival rate).
 """
 import argparse
 import json
@@ -27,6 +35,55 @@ BYTES_PER_CELL = 152.0
 
 WORKLOAD = dict(domain=(256, 256, 256), nums=(2, 2, 2), cs=0.17, lid=0.1, tau=0.5005, amp=1e-4,
                 seed=20260101)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernel_256cube_les_2x2x2_hbm_traffic.txt")
+
+
+def measured_traffic(exchange_mode, kernel_variant, wg):
+    """HBM bytes per step-kernel launch from the committed counter summary,
+    if it was measured with this launch configuration (else None)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    cfg, val = None, None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("config:"):
+            cfg = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+        elif line.startswith("traffic_bytes_per_launch:"):
+            val = float(line.split(":", 1)[1])
+    if cfg != {"exchange_mode": str(exchange_mode), "kernel_variant": str(kernel_variant), "work_group": str(wg)}:
+        return None
+    return val
+
+
+def batching_leg(log):
+    from turbulent_lbm_multigpu_amd import batch
+
+    cases = batch.make_cavities(512, 20260101)
+    b, _ = batch.run_batched(cases, 1000, autotune_pairs=2)
+    s, _ = batch.run_sequential(cases, 1000)
+    log(f"[bench] batching: {b['cases_per_s']} cases/s batched, {s['cases_per_s']} one at a time")
+    return {"workload": "512 LES cavities, edges U{16..96}, lid U[0.02,0.1], tau lognormal(0.52,0.01) >= 0.5002, "
+                        "Cs 0.17, FP32, 1000 steps, seed 20260101",
+            "cases_per_s": b["cases_per_s"], "mlups": b["mlups"], "wall_s": b["wall_s"], "tune_s": b["tune_s"],
+            "launch": b["launch"], "one_at_a_time_cases_per_s": s["cases_per_s"],
+            "speedup": round(b["cases_per_s"] / s["cases_per_s"], 3)}
+
+
+def serving_leg(log, requests=24, rate=4.0):
+    from turbulent_lbm_multigpu_amd import serving
+
+    sat = serving.compare(serving.request_stream(requests, 0.0, 20260101), 200e9)
+    log(f"[bench] serving at saturation: {json.dumps(sat)}")
+    lat = serving.compare(serving.request_stream(requests, rate, 20260101), 200e9)
+    log(f"[bench] serving at {rate}/s: {json.dumps(lat)}")
+    b = lat["fp16_batched"]
+    return {"workload": f"{requests} cavity/channel requests, cubic edge log-uniform 32-192, steps U{{500..5000}}, "
+                        "200 GB budget, seed 20260101",
+            "cases_per_s": sat["fp16_batched"]["cases_per_s"],
+            "fp32_one_at_a_time_cases_per_s": sat["fp32_one_at_a_time"]["cases_per_s"],
+            "speedup_cases_per_s": sat["speedup_cases_per_s"],
+            "rate_per_s": rate, "p50_latency_s": b["p50_latency_s"], "p99_latency_s": b["p99_latency_s"],
+            "fp32_one_at_a_time_p50_latency_s": lat["fp32_one_at_a_time"]["p50_latency_s"],
+            "fp32_one_at_a_time_p99_latency_s": lat["fp32_one_at_a_time"]["p99_latency_s"]}
 
 
 def build_group(exchange_mode=0, graph=True, wg=256, plane_padding=-1, kernel_variant=0, lds_per_block=0):
@@ -121,7 +178,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--timing-steps", type=int, default=200)
-    ap.add_argument("--exchange-mode", type=int, default=0)
+    ap.add_argument("--exchange-mode", type=int, default=1)
     ap.add_argument("--wg", type=int, default=256)
     ap.add_argument("--plane-padding", type=int, default=-1)
     ap.add_argument("--kernel-variant", type=int, default=0)
@@ -129,6 +186,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -169,6 +228,16 @@ def main():
     copy_gbs = copy_probe(lib)
     mass, ke = grp.solvers[0].mass_energy()
 
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    traffic = measured_traffic(args.exchange_mode, args.kernel_variant, args.wg)
+    grp.close()
+    for s in grp.solvers:
+        s.close()
+    batching = None if args.no_batch else batching_leg(log)
+    serving_res = None if args.no_serving else serving_leg(log)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
@@ -202,7 +271,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": round(traffic) if traffic else None,
         },
         "cpu_baseline": cpu,
         "step_kernel_us": round(kern_ms * 1e3, 2),
@@ -211,9 +280,10 @@ def main():
         "copy_probe_gbs": round(copy_gbs, 1),
         "exchange_kernel_us": round(tm["exchange_ms"] * 1e3, 2),
         "subdomain0_mass_ke": [mass, ke],
+        "batching": batching,
+        "serving": serving_res,
     }
     print(json.dumps(out), flush=True)
-    grp.close()
 
 
 if __name__ == "__main__":
