@@ -86,7 +86,7 @@ def serving_leg(log, requests=24, rate=4.0):
             "fp32_one_at_a_time_p99_latency_s": lat["fp32_one_at_a_time"]["p99_latency_s"]}
 
 
-def build_group(exchange_mode=0, graph=True, wg=256, plane_padding=-1, kernel_variant=0, lds_per_block=0):
+def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_block=0):
     import numpy as np
 
     from turbulent_lbm_multigpu_amd import _capi
@@ -100,7 +100,7 @@ def build_group(exchange_mode=0, graph=True, wg=256, plane_padding=-1, kernel_va
     for sd in subs:
         bc = [sd.bc[0][0], sd.bc[0][1], sd.bc[1][0], sd.bc[1][1], sd.bc[2][0], sd.bc[2][1]]
         s = LbmSolver(sd.domain.size, bc, inv_tau=1.0 / w["tau"], tau=w["tau"], smagorinsky_cs=w["cs"],
-                      lid_velocity=w["lid"], work_group_size=wg, plane_padding=plane_padding,
+                      lid_velocity=w["lid"], work_group_size=wg,
                       kernel_variant=kernel_variant, lds_per_block=lds_per_block)
         s.init_perturbed(w["amp"], w["seed"], 0, dec.key_origin(sd), mono)
         if sd.has_lid:
@@ -180,7 +180,6 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=200)
     ap.add_argument("--exchange-mode", type=int, default=1)
     ap.add_argument("--wg", type=int, default=256)
-    ap.add_argument("--plane-padding", type=int, default=-1)
     ap.add_argument("--kernel-variant", type=int, default=0)
     ap.add_argument("--lds-per-block", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
@@ -200,7 +199,7 @@ def main():
     lib = _capi.load_library()
     print(f"[bench] building 256^3 LES 2x2x2 (exchange_mode={args.exchange_mode}, wg={args.wg})",
           file=sys.stderr, flush=True)
-    grp = build_group(args.exchange_mode, not args.no_graph, args.wg, args.plane_padding, args.kernel_variant,
+    grp = build_group(args.exchange_mode, not args.no_graph, args.wg, args.kernel_variant,
                       args.lds_per_block)
     grp.step(args.warmup)
     grp.wait()
@@ -263,7 +262,7 @@ def main():
             "cs": WORKLOAD["cs"], "lid": WORKLOAD["lid"], "tau0": WORKLOAD["tau"],
             "init": "f_eq(1, du), du ~ U(+-1e-4), seed 20260101",
             "exchange_mode": args.exchange_mode, "work_group": args.wg, "graph": not args.no_graph,
-            "kernel_variant": args.kernel_variant, "plane_padding": args.plane_padding,
+            "kernel_variant": args.kernel_variant,
         },
         "roofline": {
             "bound": "hbm",

@@ -81,7 +81,6 @@ typedef struct lbm_solver_config {
     int store_velocity;     /* kernels write the velocity field (STORE_VELOCITY) */
     int store_density;      /* kernels write the density field (STORE_DENSITY) */
     int work_group_size;    /* threads per work-group of the step kernels: 64..256; 0 = 256 */
-    int plane_padding;      /* extra elements per distribution plane; -1 = default */
     int device;             /* HIP device ordinal */
     int kernel_variant;     /* tuning bits: 1 non-temporal loads, 2 non-temporal stores, 4 no XCD tile map */
     int lds_per_block;      /* bytes of LDS reserved per work-group (caps residency); 0 = none */
@@ -165,8 +164,14 @@ int lbm_copy_box(lbm_solver *src, const int src_origin[3], lbm_solver *dst, cons
  * of it; FLUID cells inside become OBSTACLE.  achieved = covered fraction. */
 int lbm_solver_place_spheres(lbm_solver *s, uint64_t seed, double fill, double rmin, double rmax,
                              const int key_origin[3], const int key_dims[3], double *achieved);
-/* raw device pointers (for callers that bring their own kernels) */
-int lbm_solver_device_pointers(lbm_solver *s, void **dd, void **flags, long long *plane_stride);
+/* raw device pointers (for callers that bring their own kernels).  The
+ * distributions are stored in blocks of `block_cells` consecutive cells (cell
+ * order x + X*(y + Y*z)), each block holding its 19 populations one after the
+ * other: population i of cell g is element
+ *   (g / block_cells) * 19 * block_cells + i * block_cells + g % block_cells
+ * (the reference's [19][cells] planes, src/CLbmSolver.hpp:381-382, blocked);
+ * flags are one byte per cell. */
+int lbm_solver_device_pointers(lbm_solver *s, void **dd, void **flags, long long *block_cells);
 
 /* ---- group: N subdomains stepped together on one device ----------------- */
 /* One fused step kernel over all member solvers per half-step, then the

@@ -90,7 +90,6 @@ class SolverConfig(C.Structure):
         ("store_velocity", C.c_int),
         ("store_density", C.c_int),
         ("work_group_size", C.c_int),
-        ("plane_padding", C.c_int),
         ("device", C.c_int),
         ("kernel_variant", C.c_int),
         ("lds_per_block", C.c_int),

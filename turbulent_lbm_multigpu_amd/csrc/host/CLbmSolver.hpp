@@ -54,7 +54,6 @@ struct CLatticeOverrides {
     int forcing = LBM_FORCING_REFERENCE;
     int storage = -1;         // -1: from T (f32 / f64)
     int kernel_variant = 0;
-    int plane_padding = -1;
 };
 
 }  // namespace lbm
@@ -167,7 +166,6 @@ public:
         if (it != lbm_opencl_number_of_work_items_list.end() && *it != 0) wg = *it;
         c.work_group_size = wg;
         c.kernel_variant = _lattice.kernel_variant;
-        c.plane_padding = _lattice.plane_padding;
         c.device = _ctx.device;
         if (fail(lbm_solver_create(&c, &_h))) return;
         simulation_step_counter = 0;

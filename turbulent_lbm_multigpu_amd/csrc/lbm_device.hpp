@@ -65,15 +65,35 @@ __host__ __device__ constexpr T weight(int i) {
 
 // ---- storage of the distributions -------------------------------------------
 // 0: f32, 1: f64, 2: f16 holding f - w, 3: bf16 holding f - w.
+//
+// Layout in HBM: blocks of kBlk = 4096 consecutive cells (cell order
+// g = x + X*(y + Y*z)) hold their 19 populations one after the other, so
+// population i of cell g is element
+//     (g / kBlk) * 19 * kBlk + i * kBlk + g % kBlk.
+// Inside a block each population is a contiguous run of cells, so a wave's
+// accesses are as coalesced as in a [19][cells] layout, but the 38 streams of
+// a work-group stay inside one 19 * kBlk element window instead of 19 planes
+// a whole domain apart (on MI355X the in-place 19-load/19-store pattern over
+// eight 128^3 domains moves ~20% more bytes per second this way).
+//
 // Accesses go through address-space-1 (global) pointers: a uniform 64-bit
-// plane base in SGPRs plus a 32-bit per-lane byte offset, which the compiler
-// turns into `global_load/store ... v_off, s[base]` (no 64-bit VALU address
-// arithmetic, no flat instructions).
+// population base (dd + i * kBlk elements) in SGPRs plus a 32-bit per-lane
+// byte offset (cell_base(g) elements), which the compiler turns into
+// `global_load/store ... v_off, s[base]` (no 64-bit VALU address arithmetic,
+// no flat instructions).  The whole array of one domain therefore stays below
+// 4 GiB.
 typedef __attribute__((address_space(1))) char gchar;
 typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) unsigned short gushort;
 typedef __attribute__((address_space(1))) unsigned char guchar;
+
+constexpr unsigned kBlkShift = 12;
+constexpr unsigned kBlk = 1u << kBlkShift;
+// element offset of cell g from its population base
+__host__ __device__ __forceinline__ unsigned cell_base(unsigned g) {
+    return (g >> kBlkShift) * (19u * kBlk) + (g & (kBlk - 1u));
+}
 
 __device__ __forceinline__ unsigned short bf16_bits_rne(float v) {
     unsigned u = __float_as_uint(v);
@@ -133,7 +153,7 @@ struct Storage<3> {
     __device__ static U to_bits(T v, int i) { return bf16_bits_rne(v - weight<float>(i)); }
 };
 
-// one element of population slot i at byte offset `off` of its plane
+// one element of population slot i at byte offset `off` from its population base
 template <class SP, bool NT = false>
 __device__ __forceinline__ typename SP::T ld(const gchar *base, unsigned off, int i) {
     return SP::from_bits(raw_ld<NT, typename SP::U>(base, off), i);
@@ -169,11 +189,10 @@ __host__ __device__ constexpr int cells_per_lane(int var) { return 1 << ((var >>
 
 // ---- per-(sub)domain launch descriptor ---------------------------------------
 struct DomDev {
-    void *dd;          // 19 planes of `plane` elements
+    void *dd;          // blocks of kBlk cells x 19 populations
     uint8_t *flags;    // one byte per cell
     void *vel;         // [3][cells] compute type, or null
     void *dens;        // [cells] compute type, or null
-    long long plane;   // plane stride in elements
     long long cells;
     int nx, ny, nz;
     int tile_begin;    // first tile of this domain in the launch
@@ -448,19 +467,20 @@ __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int ce
                                             const DomDev *D) {
     using T = typename SP::T;
     const int flag = ((const guchar *)D->flags)[g];
+    const unsigned own = cell_base(g) * SP::kBytes;
     T f[19];
     if (BETA) {
 #pragma unroll
         for (int i = 0; i < 18; ++i) {
             const unsigned idx = wrap ? (unsigned)wrap_idx((int)g - d[i], cells) : g - (unsigned)d[i];
-            f[i] = ld<SP>(pl[opp(i)], idx * SP::kBytes, opp(i));
+            f[i] = ld<SP>(pl[opp(i)], cell_base(idx) * SP::kBytes, opp(i));
         }
-        f[18] = ld<SP>(pl[18], g * SP::kBytes, 18);
+        f[18] = ld<SP>(pl[18], own, 18);
     } else {
         // the 19 loads do not wait for the flag; a GHOST cell (lbm_alpha.cl:30-32)
         // matches no branch of cell_update and stores nothing
 #pragma unroll
-        for (int i = 0; i < 19; ++i) f[i] = ld<SP>(pl[i], g * SP::kBytes, i);
+        for (int i = 0; i < 19; ++i) f[i] = ld<SP>(pl[i], own, i);
     }
     T vx, vy, vz, dens;
     const bool write_dd = cell_update<T, BETA>(f, flag, D, vx, vy, vz, dens);
@@ -469,20 +489,43 @@ __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int ce
 #pragma unroll
             for (int i = 0; i < 18; ++i) {
                 const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
-                st<SP, NTS>(pl[i], idx * SP::kBytes, i, f[i]);
+                st<SP, NTS>(pl[i], cell_base(idx) * SP::kBytes, i, f[i]);
             }
-            st<SP, NTS>(pl[18], g * SP::kBytes, 18, f[18]);
+            st<SP, NTS>(pl[18], own, 18, f[18]);
         } else {
 #pragma unroll
-            for (int i = 0; i < 19; ++i) st<SP, NTS>(pl[opp(i)], g * SP::kBytes, opp(i), f[i]);
+            for (int i = 0; i < 19; ++i) st<SP, NTS>(pl[opp(i)], own, opp(i), f[i]);
         }
     }
     if (flag == kFlagGhost) return;  // lbm_alpha.cl:30-32, lbm_beta.cl:787-788
     store_macros<T>(D, g, cells, vx, vy, vz, dens);
 }
 
-// N x-consecutive cells g0 .. g0+N-1 away from the domain ends: each of the
-// 19 (beta: 18 shifted + 1) plane accesses is one N-element vector access.
+// N cells s .. s+N-1 of population base p: one N-element vector access when
+// they lie in one layout block, element accesses when they straddle two.
+template <class SP, int N>
+__device__ __forceinline__ void vld_any(const gchar *p, unsigned s, int i, typename SP::T *out) {
+    if ((s & (kBlk - 1u)) <= kBlk - (unsigned)N) {
+        vld<SP, N>(p, cell_base(s) * SP::kBytes, i, out);
+    } else {
+#pragma unroll
+        for (int j = 0; j < N; ++j) out[j] = ld<SP>(p, cell_base(s + j) * SP::kBytes, i);
+    }
+}
+template <class SP, int N>
+__device__ __forceinline__ void vst_any(gchar *p, unsigned s, int i, const typename SP::T *in) {
+    if ((s & (kBlk - 1u)) <= kBlk - (unsigned)N) {
+        vst<SP, N>(p, cell_base(s) * SP::kBytes, i, in);
+    } else {
+#pragma unroll
+        for (int j = 0; j < N; ++j) st<SP>(p, cell_base(s + j) * SP::kBytes, i, in[j]);
+    }
+}
+
+// N x-consecutive cells g0 .. g0+N-1 away from the domain ends (g0 a multiple
+// of N, so they share a layout block): each of the 19 (beta: 18 shifted + 1)
+// population accesses is one N-element vector access, or N element accesses
+// where a shifted run straddles two blocks.
 template <class SP, bool BETA, int N>
 __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int cells, const int *d,
                                              const DomDev *D) {
@@ -496,12 +539,13 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
 #pragma unroll
         for (int j = 0; j < N; ++j) flag[j] = fv[j];
     }
+    const unsigned own = cell_base(g0) * es;
     T F[19][N];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const int src = BETA ? opp(i) : i;
-        const unsigned off = BETA ? (g0 - (unsigned)d[i]) * es : g0 * es;
-        vld<SP, N>(pl[src], off, src, F[i]);
+        if (BETA && i < 18) vld_any<SP, N>(pl[src], g0 - (unsigned)d[i], src, F[i]);
+        else vld<SP, N>(pl[src], own, src, F[i]);
     }
     T vx[N], vy[N], vz[N], dn[N];
     bool all_write = true, any_ghost = false;
@@ -519,16 +563,17 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
     }
     if (BETA) {
 #pragma unroll
-        for (int i = 0; i < 19; ++i) vst<SP, N>(pl[i], (g0 + (unsigned)d[i]) * es, i, F[i]);
+        for (int i = 0; i < 18; ++i) vst_any<SP, N>(pl[i], g0 + (unsigned)d[i], i, F[i]);
+        vst<SP, N>(pl[18], own, 18, F[18]);
     } else if (all_write) {
 #pragma unroll
-        for (int i = 0; i < 19; ++i) vst<SP, N>(pl[opp(i)], g0 * es, opp(i), F[i]);
+        for (int i = 0; i < 19; ++i) vst<SP, N>(pl[opp(i)], own, opp(i), F[i]);
     } else {
 #pragma unroll
         for (int j = 0; j < N; ++j)
             if (flag[j] >= 0)
 #pragma unroll
-                for (int i = 0; i < 19; ++i) st<SP>(pl[opp(i)], (g0 + j) * es, opp(i), F[i][j]);
+                for (int i = 0; i < 19; ++i) st<SP>(pl[opp(i)], own + j * es, opp(i), F[i][j]);
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
@@ -565,9 +610,8 @@ __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ do
     gchar *pl[19];
     {
         gchar *base = (gchar *)D->dd;
-        const size_t pb = (size_t)D->plane * SP::kBytes;
 #pragma unroll
-        for (int i = 0; i < 19; ++i) pl[i] = base + (size_t)i * pb;
+        for (int i = 0; i < 19; ++i) pl[i] = base + (size_t)i * kBlk * SP::kBytes;
     }
     int d[19];
     bool wrap = false;

@@ -48,10 +48,15 @@ __device__ __forceinline__ double unit01(uint64_t seed, uint64_t case_id, uint64
     return (double)(h >> 11) * (1.0 / 9007199254740992.0);
 }
 
-// plane base of population i (global address space) and byte offset of cell g
+// base of population i (global address space); cell g sits cell_base(g)
+// elements further
 template <class SP>
-__device__ __forceinline__ gchar *plane_ptr(const DomDev &D, int i) {
-    return (gchar *)D.dd + (size_t)i * (size_t)D.plane * SP::kBytes;
+__device__ __forceinline__ gchar *pop_ptr(const DomDev &D, int i) {
+    return (gchar *)D.dd + (size_t)i * kBlk * SP::kBytes;
+}
+template <class SP>
+__device__ __forceinline__ unsigned cell_off(long long g) {
+    return cell_base((unsigned)g) * SP::kBytes;
 }
 
 struct InitArgs {
@@ -92,7 +97,7 @@ __global__ void init_kernel(DomDev D, InitArgs a) {
     T f[19];
     feq19(dd_param, u[0], u[1], u[2], f);
 #pragma unroll
-    for (int i = 0; i < 19; ++i) st<SP>(plane_ptr<SP>(D, opp(i)), (unsigned)g * SP::kBytes, opp(i), f[i]);
+    for (int i = 0; i < 19; ++i) st<SP>(pop_ptr<SP>(D, opp(i)), cell_off<SP>(g), opp(i), f[i]);
     if (D.vel) {
         T *v = (T *)D.vel;
         v[g] = u[0];
@@ -122,7 +127,7 @@ __global__ void pack_kernel(DomDev D, int field, Box b, void *out) {
     if (field == LBM_FIELD_DD) {
         T *o = (T *)out;
 #pragma unroll
-        for (int i = 0; i < 19; ++i) o[i * n + e] = ld<SP>(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i);
+        for (int i = 0; i < 19; ++i) o[i * n + e] = ld<SP>(pop_ptr<SP>(D, i), cell_off<SP>(g), i);
     } else if (field == LBM_FIELD_VELOCITY) {
         const T *v = (const T *)D.vel;
         T *o = (T *)out;
@@ -147,7 +152,7 @@ __global__ void unpack_kernel(DomDev D, int field, Box b, const void *in, uint32
         const T *s = (const T *)in;
 #pragma unroll
         for (int i = 0; i < 19; ++i)
-            if (mask & (1u << i)) st<SP>(plane_ptr<SP>(D, i), (unsigned)g * SP::kBytes, i, s[i * n + e]);
+            if (mask & (1u << i)) st<SP>(pop_ptr<SP>(D, i), cell_off<SP>(g), i, s[i * n + e]);
     } else if (field == LBM_FIELD_VELOCITY) {
         T *v = (T *)D.vel;
         const T *s = (const T *)in;
@@ -166,7 +171,6 @@ __global__ void unpack_kernel(DomDev D, int field, Box b, const void *in, uint32
 struct CopyOp {
     const void *src;
     void *dst;
-    long long src_plane, dst_plane;  // in elements
     int src_nx, src_ny, dst_nx, dst_ny;
     int so[3], dof[3], sz[3];
     uint32_t mask;
@@ -197,16 +201,15 @@ __global__ void __launch_bounds__(256) exchange_kernel(const CopyOp *__restrict_
     const unsigned gs = (unsigned)((op.so[0] + bx) + op.src_nx * ((op.so[1] + by) + op.src_ny * (op.so[2] + bz)));
     const unsigned gd = (unsigned)((op.dof[0] + bx) + op.dst_nx * ((op.dof[1] + by) + op.dst_ny * (op.dof[2] + bz)));
     const int i = op.pops[p];
-    const gchar *sp = (const gchar *)op.src + (size_t)i * (size_t)op.src_plane * B;
-    gchar *dp = (gchar *)op.dst + (size_t)i * (size_t)op.dst_plane * B;
+    const gchar *sp = (const gchar *)op.src + (size_t)i * kBlk * B + cell_base(gs) * B;
+    gchar *dp = (gchar *)op.dst + (size_t)i * kBlk * B + cell_base(gd) * B;
     if (B == 8)
-        *(__attribute__((address_space(1))) unsigned long long *)(dp + gd * B) =
-            *(const __attribute__((address_space(1))) unsigned long long *)(sp + gs * B);
+        *(__attribute__((address_space(1))) unsigned long long *)dp =
+            *(const __attribute__((address_space(1))) unsigned long long *)sp;
     else if (B == 4)
-        *(__attribute__((address_space(1))) unsigned *)(dp + gd * B) =
-            *(const __attribute__((address_space(1))) unsigned *)(sp + gs * B);
+        *(__attribute__((address_space(1))) unsigned *)dp = *(const __attribute__((address_space(1))) unsigned *)sp;
     else
-        *(gushort *)(dp + gd * B) = *(const gushort *)(sp + gs * B);
+        *(gushort *)dp = *(const gushort *)sp;
 }
 
 // rho and j = sum f_i e_i read in the current step parity: after an even
@@ -219,7 +222,7 @@ __device__ __forceinline__ void cell_moments(const DomDev &D, long long g, bool 
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const int slot = swapped ? opp(i) : i;
-        const double v = (double)ld<SP>(plane_ptr<SP>(D, slot), (unsigned)g * SP::kBytes, slot);
+        const double v = (double)ld<SP>(pop_ptr<SP>(D, slot), cell_off<SP>(g), slot);
         r += v;
         a += v * ex(i);
         bb += v * ey(i);
@@ -330,7 +333,7 @@ uint32_t norm_mask(const int *norm) {
 
 struct lbm_solver {
     lbm_solver_config cfg{};
-    long long cells = 0, plane = 0;
+    long long cells = 0, padded = 0;  // padded: cells rounded up to whole layout blocks
     int wg = 256;
     void *dd = nullptr;
     uint8_t *flags = nullptr;
@@ -429,7 +432,6 @@ void fill_desc(lbm_solver *s) {
     d.flags = s->flags;
     d.vel = s->vel;
     d.dens = s->dens;
-    d.plane = s->plane;
     d.cells = s->cells;
     d.nx = s->cfg.dims[0];
     d.ny = s->cfg.dims[1];
@@ -549,7 +551,6 @@ extern "C" void lbm_solver_config_default(lbm_solver_config *c) {
     c->tau = 0.953575;  // src/CLbmSkeleton.hpp:142
     c->inv_tau = 1.0 / 0.953575;
     c->work_group_size = 256;
-    c->plane_padding = -1;
     c->device = 0;
 }
 
@@ -564,13 +565,13 @@ struct SolverLayout {
     size_t dd = 0, flags = 0, vel = 0, dens = 0, desc = 0, tile = 0, staging = 0, staging_bytes = 0, total = 0;
 };
 
-static SolverLayout solver_layout(const lbm_solver_config &c, long long cells, long long plane, int ntiles) {
+static SolverLayout solver_layout(const lbm_solver_config &c, long long cells, long long padded, int ntiles) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t es = elem_size(c.storage), cs = compute_size(c.storage);
     SolverLayout L;
     size_t off = 0;
     L.dd = off;
-    off += al((size_t)19 * plane * es);
+    off += al((size_t)19 * padded * es);
     L.flags = off;
     off += al((size_t)cells);
     if (c.store_velocity) {
@@ -607,10 +608,11 @@ static int validate_config(const lbm_solver_config &c) {
         if (c.bc[i] < 0 || c.bc[i] > 255)
             return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "boundary flags must be in [0, 255]");
     if (c.storage < 0 || c.storage > 3) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "unknown storage");
-    // kernels address a plane with 32-bit byte offsets
-    if ((unsigned long long)cells * elem_size(c.storage) >= (1ull << 32))
+    // kernels address the distributions of a domain with 32-bit byte offsets
+    const unsigned long long padded = (unsigned long long)(cells + kBlk - 1) / kBlk * kBlk;
+    if (19ull * padded * elem_size(c.storage) >= (1ull << 32))
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
-                      "one distribution plane must stay below 4 GiB: split the domain into subdomains");
+                      "the distributions of one (sub)domain must stay below 4 GiB: split the domain into subdomains");
     if (c.forcing != 0 && c.forcing != 1) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "unknown forcing");
     const int wg = c.work_group_size == 0 ? 256 : c.work_group_size;
     if (wg < 64 || wg > 256 || wg % 64 != 0)
@@ -627,10 +629,9 @@ static int validate_config(const lbm_solver_config &c) {
     return LBM_OK;
 }
 
-static void solver_geometry(const lbm_solver_config &c, long long &cells, long long &plane, int &wg, int &ntiles) {
+static void solver_geometry(const lbm_solver_config &c, long long &cells, long long &padded, int &wg, int &ntiles) {
     cells = (long long)c.dims[0] * c.dims[1] * c.dims[2];
-    const long long pad = c.plane_padding < 0 ? 0 : c.plane_padding;
-    plane = ((cells + 255) / 256) * 256 + pad;
+    padded = (cells + kBlk - 1) / kBlk * kBlk;
     wg = c.work_group_size == 0 ? 256 : c.work_group_size;
     const long long per_tile = (long long)wg * cells_per_lane(c.kernel_variant);
     ntiles = (int)((cells + per_tile - 1) / per_tile);
@@ -643,10 +644,10 @@ static int create_solver(const lbm_solver_config *cfg, void *memory, size_t memo
     const lbm_solver_config &c = *cfg;
     int rc0 = validate_config(c);
     if (rc0) return rc0;
-    long long cells, plane;
+    long long cells, padded;
     int wg, ntiles;
-    solver_geometry(c, cells, plane, wg, ntiles);
-    const SolverLayout L = solver_layout(c, cells, plane, ntiles);
+    solver_geometry(c, cells, padded, wg, ntiles);
+    const SolverLayout L = solver_layout(c, cells, padded, ntiles);
     if (memory && (memory_bytes < L.total || ((uintptr_t)memory & 255) != 0))
         return record(nullptr, LBM_ERR_CAPACITY,
                       "device memory block of " + std::to_string(memory_bytes) + " bytes (256-byte aligned) is smaller than the " +
@@ -657,7 +658,7 @@ static int create_solver(const lbm_solver_config *cfg, void *memory, size_t memo
     s->cfg.work_group_size = wg;
     s->wg = wg;
     s->cells = cells;
-    s->plane = plane;
+    s->padded = padded;
     s->ntiles = ntiles;
     s->external = memory != nullptr;
     auto bail = [&](int code, const std::string &m) {
@@ -685,7 +686,7 @@ static int create_solver(const lbm_solver_config *cfg, void *memory, size_t memo
         s->staging_bytes = L.staging_bytes;
     } else {
         const size_t es = elem_size(c.storage), cs = compute_size(c.storage);
-        if (hipMalloc(&s->dd, (size_t)19 * s->plane * es) != hipSuccess)
+        if (hipMalloc(&s->dd, (size_t)19 * s->padded * es) != hipSuccess)
             return bail(LBM_ERR_OUT_OF_MEMORY, "distribution buffer allocation failed");
         if (hipMalloc((void **)&s->flags, (size_t)cells) != hipSuccess)
             return bail(LBM_ERR_OUT_OF_MEMORY, "flag buffer allocation failed");
@@ -716,10 +717,10 @@ extern "C" int lbm_solver_required_bytes(const lbm_solver_config *cfg, size_t *b
     if (!cfg || !bytes) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
     int rc = validate_config(*cfg);
     if (rc) return rc;
-    long long cells, plane;
+    long long cells, padded;
     int wg, ntiles;
-    solver_geometry(*cfg, cells, plane, wg, ntiles);
-    *bytes = solver_layout(*cfg, cells, plane, ntiles).total;
+    solver_geometry(*cfg, cells, padded, wg, ntiles);
+    *bytes = solver_layout(*cfg, cells, padded, ntiles).total;
     return LBM_OK;
 }
 
@@ -986,11 +987,11 @@ extern "C" int lbm_solver_moments(lbm_solver *s, const int origin[3], const int 
     return LBM_OK;
 }
 
-extern "C" int lbm_solver_device_pointers(lbm_solver *s, void **dd, void **flags, long long *plane) {
+extern "C" int lbm_solver_device_pointers(lbm_solver *s, void **dd, void **flags, long long *block_cells) {
     if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null solver");
     if (dd) *dd = s->dd;
     if (flags) *flags = s->flags;
-    if (plane) *plane = s->plane;
+    if (block_cells) *block_cells = kBlk;
     return LBM_OK;
 }
 
@@ -1005,8 +1006,6 @@ static int fill_op(lbm_solver *src, const int *so, lbm_solver *dst, const int *d
     }
     op.src = src->dd;
     op.dst = dst->dd;
-    op.src_plane = src->plane;
-    op.dst_plane = dst->plane;
     op.src_nx = src->cfg.dims[0];
     op.src_ny = src->cfg.dims[1];
     op.dst_nx = dst->cfg.dims[0];

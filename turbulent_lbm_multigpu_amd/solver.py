@@ -44,7 +44,7 @@ class LbmSolver:
                  lid_velocity: float = 0.0, gravitation: Sequence[float] = (0.0, 0.0, 0.0),
                  storage: int = _capi.STORAGE_F32, forcing: int = _capi.FORCING_REFERENCE,
                  store_velocity: bool = False, store_density: bool = False,
-                 work_group_size: int = 256, plane_padding: int = -1, device: int = 0,
+                 work_group_size: int = 256, device: int = 0,
                  kernel_variant: int = 0, lds_per_block: int = 0):
         self._lib = _capi.load_library()
         cfg = SolverConfig()
@@ -64,7 +64,6 @@ class LbmSolver:
         cfg.store_velocity = int(bool(store_velocity))
         cfg.store_density = int(bool(store_density))
         cfg.work_group_size = int(work_group_size)
-        cfg.plane_padding = int(plane_padding)
         cfg.device = int(device)
         cfg.kernel_variant = int(kernel_variant)
         cfg.lds_per_block = int(lds_per_block)
