@@ -38,7 +38,7 @@ WORKLOAD = dict(domain=(256, 256, 256), nums=(2, 2, 2), cs=0.17, lid=0.1, tau=0.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernel_256cube_les_2x2x2_hbm_traffic.txt")
 
 
-def measured_traffic(exchange_mode, kernel_variant, wg):
+def measured_traffic(exchange_mode, kernel_variant, wg, fused_x):
     """HBM bytes per step-kernel launch from the committed counter summary,
     if it was measured with this launch configuration (else None)."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -49,7 +49,8 @@ def measured_traffic(exchange_mode, kernel_variant, wg):
             cfg = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
         elif line.startswith("traffic_bytes_per_launch:"):
             val = float(line.split(":", 1)[1])
-    if cfg != {"exchange_mode": str(exchange_mode), "kernel_variant": str(kernel_variant), "work_group": str(wg)}:
+    if cfg != {"exchange_mode": str(exchange_mode), "kernel_variant": str(kernel_variant), "work_group": str(wg),
+               "fused_x": str(int(fused_x))}:
         return None
     return val
 
@@ -86,7 +87,7 @@ def serving_leg(log, requests=24, rate=4.0):
             "fp32_one_at_a_time_p99_latency_s": lat["fp32_one_at_a_time"]["p99_latency_s"]}
 
 
-def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_block=0):
+def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_block=0, fused_x=True):
     import numpy as np
 
     from turbulent_lbm_multigpu_amd import _capi
@@ -112,6 +113,7 @@ def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_b
         for c in sd.comms:
             grp.add_comm(sd.uid, c.dst_id, c.send_origin, c.recv_origin, c.send_size, c.comm_direction)
     grp.set_exchange_mode(exchange_mode)
+    grp.set_fused_x(fused_x)
     grp.set_graph(graph)
     return grp
 
@@ -183,6 +185,8 @@ def main():
     ap.add_argument("--kernel-variant", type=int, default=0)
     ap.add_argument("--lds-per-block", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--fused-x", type=int, default=1, choices=[0, 1],
+                    help="exchange the x faces inside the step kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-batch", action="store_true")
@@ -200,7 +204,7 @@ def main():
     print(f"[bench] building 256^3 LES 2x2x2 (exchange_mode={args.exchange_mode}, wg={args.wg})",
           file=sys.stderr, flush=True)
     grp = build_group(args.exchange_mode, not args.no_graph, args.wg, args.kernel_variant,
-                      args.lds_per_block)
+                      args.lds_per_block, bool(args.fused_x))
     grp.step(args.warmup)
     grp.wait()
     lib.lbm_device_synchronize(0)
@@ -230,7 +234,7 @@ def main():
     def log(m):
         print(m, file=sys.stderr, flush=True)
 
-    traffic = measured_traffic(args.exchange_mode, args.kernel_variant, args.wg)
+    traffic = measured_traffic(args.exchange_mode, args.kernel_variant, args.wg, args.fused_x)
     grp.close()
     for s in grp.solvers:
         s.close()
@@ -262,7 +266,7 @@ def main():
             "cs": WORKLOAD["cs"], "lid": WORKLOAD["lid"], "tau0": WORKLOAD["tau"],
             "init": "f_eq(1, du), du ~ U(+-1e-4), seed 20260101",
             "exchange_mode": args.exchange_mode, "work_group": args.wg, "graph": not args.no_graph,
-            "kernel_variant": args.kernel_variant,
+            "kernel_variant": args.kernel_variant, "fused_x": bool(args.fused_x),
         },
         "roofline": {
             "bound": "hbm",

@@ -290,3 +290,51 @@ def test_shifted_storage_error_vs_fp64(storage):
     assert e_u < bound_u
     ref.close()
     low.close()
+
+
+def fluid_slots(s: LbmSolver):
+    """The 19 slots of every FLUID cell (what the next half-step reads)."""
+    fl = s.store(_capi.FIELD_FLAGS).reshape(-1)
+    return gpu_dd(s).reshape(19, -1)[:, fl == orc.FLAG_FLUID]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("domain_size,nums", [((28, 28, 28), (2, 2, 2)), ((36, 16, 24), (3, 1, 2))])
+def test_fused_x_faces_match_copy_exchange(mode, domain_size, nums):
+    """x faces exchanged inside the step kernel: every FLUID cell's slots are
+    bit-identical to the copy-based exchange, after odd and even step counts."""
+    say(f"case: fused x faces {domain_size} {nums} mode {mode}")
+    kw = dict(inv_tau=1 / 0.52, tau=0.52, smagorinsky_cs=0.17)
+    ref, subs, _ = build_gpu_group(domain_size, nums, lid=0.08, amp=1e-3, seed=11, mode=mode, **kw)
+    fus, _, _ = build_gpu_group(domain_size, nums, lid=0.08, amp=1e-3, seed=11, mode=mode, **kw)
+    fus.set_fused_x(True)
+    for steps in (5, 8):
+        ref.step(steps)
+        fus.step(steps)
+        ref.wait()
+        fus.wait()
+        for a, b in zip(ref.solvers, fus.solvers):
+            np.testing.assert_array_equal(fluid_slots(a), fluid_slots(b))
+    for g in (ref, fus):
+        g.close()
+        for s in g.solvers:
+            s.close()
+
+
+def test_fused_x_periodic_channel():
+    """A periodic channel (self comms across x) with fused x faces: FLUID
+    slots equal to the copy-based exchange after 17 steps."""
+    say("case: fused x periodic channel")
+    from turbulent_lbm_multigpu_amd import precision
+
+    out = []
+    for fused in (False, True):
+        s, g, _ = precision.build_channel((24, 18, 16), _capi.STORAGE_F32, tau=0.55, gx=2e-5, fill=0.04, rmin=2.0,
+                                          rmax=3.0, amp=1e-3, seed=9)
+        g.set_fused_x(fused)
+        g.step(17)
+        g.wait()
+        out.append(fluid_slots(s))
+        g.close()
+        s.close()
+    np.testing.assert_array_equal(out[0], out[1])

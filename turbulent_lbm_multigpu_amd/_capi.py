@@ -65,6 +65,7 @@ EXPORTED_SYMBOLS = (
     "lbm_server_config_default", "lbm_case_default", "lbm_server_create", "lbm_server_destroy",
     "lbm_server_error", "lbm_server_case_bytes", "lbm_server_submit", "lbm_server_progress",
     "lbm_server_next_done", "lbm_server_status", "lbm_server_fetch", "lbm_server_stats_get", "lbm_server_time",
+    "lbm_group_set_fused_x",
 )
 
 
@@ -214,6 +215,7 @@ def load_library(path: str | None = None):
     sig("lbm_group_add_comm", I, P, I, I, IP, IP, IP, IP)
     sig("lbm_group_set_exchange_mode", I, P, I)
     sig("lbm_group_set_graph", I, P, I)
+    sig("lbm_group_set_fused_x", I, P, I)
     sig("lbm_group_step", I, P, I)
     sig("lbm_group_wait", I, P)
     sig("lbm_group_sync", I, P, I)

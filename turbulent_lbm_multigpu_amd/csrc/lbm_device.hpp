@@ -202,6 +202,12 @@ struct DomDev {
     int store_density;
     double inv_tau, tau0, les_coeff, gx, gy, gz, lid;
     long long stop_at;  // launch half-step index from which this domain stays frozen
+    // Fused x-face exchange (same-sized neighbours across x): the step kernel
+    // writes the face values straight into the neighbour instead of an
+    // exchange launch copying them afterwards.
+    void *xnb[2];          // receiver across this domain's x- / x+ face, or null
+    unsigned xmask_a[2];   // slots that receiver's ghost layer takes after alpha
+    int xrecv[2];          // this domain's x- / x+ ghost layer is fed that way
 };
 
 // ---- collision arithmetic (same operation order as the oracle) -------------
@@ -464,7 +470,7 @@ __device__ __forceinline__ void store_macros(const DomDev *D, unsigned g, int ce
 // enough from both ends of the domain for W to be the identity.
 template <class SP, bool BETA, bool NTS>
 __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
-                                            const DomDev *D) {
+                                            const DomDev *D, int x = -1) {
     using T = typename SP::T;
     const int flag = ((const guchar *)D->flags)[g];
     const unsigned own = cell_base(g) * SP::kBytes;
@@ -484,17 +490,45 @@ __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int ce
     }
     T vx, vy, vz, dens;
     const bool write_dd = cell_update<T, BETA>(f, flag, D, vx, vy, vz, dens);
+    const int X = D->nx;
     if (write_dd) {
         if (BETA) {
+            // fused x faces (x >= 0): a ghost cell of a fed face scatters no
+            // population with an x component (those slots are the sender's,
+            // and what it gathered for them may already be the sender's new
+            // value); a cell next to a feeding face also writes the receiver
+            const bool skip_x = x >= 0 && ((x == 0 && D->xrecv[0]) || (x == X - 1 && D->xrecv[1]));
 #pragma unroll
             for (int i = 0; i < 18; ++i) {
                 const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
-                st<SP, NTS>(pl[i], cell_base(idx) * SP::kBytes, i, f[i]);
+                if (!(ex(i) != 0 && skip_x)) st<SP, NTS>(pl[i], cell_base(idx) * SP::kBytes, i, f[i]);
+                if (ex(i) == 1 && x == X - 2 && D->xnb[1])
+                    st<SP>((gchar *)D->xnb[1] + (size_t)i * kBlk * SP::kBytes, cell_base(idx - X + 2) * SP::kBytes,
+                           i, f[i]);
+                if (ex(i) == -1 && x == 1 && D->xnb[0])
+                    st<SP>((gchar *)D->xnb[0] + (size_t)i * kBlk * SP::kBytes, cell_base(idx + X - 2) * SP::kBytes,
+                           i, f[i]);
             }
             st<SP, NTS>(pl[18], own, 18, f[18]);
         } else {
 #pragma unroll
             for (int i = 0; i < 19; ++i) st<SP, NTS>(pl[opp(i)], own, opp(i), f[i]);
+        }
+    }
+    if (!BETA && x >= 0) {
+        // the receiver's ghost cell across a fused x face takes this cell's
+        // slots as they stand after alpha: slot k holds f[opp(k)] if the cell
+        // stored, else its loaded value f[k]
+        const int side = x == 1 ? 0 : (x == X - 2 ? 1 : -1);
+        if (side >= 0 && D->xnb[side]) {
+            gchar *rb = (gchar *)D->xnb[side];
+            const unsigned r = side ? g - X + 2 : g + X - 2;
+            const unsigned m = D->xmask_a[side];
+#pragma unroll
+            for (int k = 0; k < 19; ++k)
+                if (m & (1u << k))
+                    st<SP>(rb + (size_t)k * kBlk * SP::kBytes, cell_base(r) * SP::kBytes, k,
+                           write_dd ? f[opp(k)] : f[k]);
         }
     }
     if (flag == kFlagGhost) return;  // lbm_alpha.cl:30-32, lbm_beta.cl:787-788
@@ -625,13 +659,25 @@ __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ do
 #pragma unroll
         for (int i = 0; i < 19; ++i) d[i] = 0;
     }
-    if (N > 1 && !wrap && g0 + N <= cells) {
+    // fused x faces: lanes holding a cell of the two outer columns on either
+    // side take the element path, which knows the x coordinate
+    const bool xf = D->xnb[0] || D->xnb[1] || D->xrecv[0] || D->xrecv[1];
+    int x0 = -1;
+    if (xf) x0 = g0 % D->nx;
+    const bool x_edge = xf && (x0 <= 1 || x0 + N - 1 >= D->nx - 2);
+    if (N > 1 && !wrap && g0 + N <= cells && !x_edge) {
         cells_vector<SP, BETA, N>(pl, (unsigned)g0, cells, d, D);
         return;
     }
 #pragma unroll
-    for (int j = 0; j < N; ++j)
-        if (g0 + j < cells) cell_scalar<SP, BETA, NTS>(pl, (unsigned)(g0 + j), cells, d, wrap, D);
+    for (int j = 0; j < N; ++j) {
+        int xj = -1;
+        if (xf) {
+            xj = x0 + j;
+            if (xj >= D->nx) xj -= D->nx;
+        }
+        if (g0 + j < cells) cell_scalar<SP, BETA, NTS>(pl, (unsigned)(g0 + j), cells, d, wrap, D, xj);
+    }
 }
 
 }  // namespace lbm

@@ -380,11 +380,19 @@ struct lbm_group {
     bool plan_built = false;
     bool descs_dirty = false;
     int exchange_mode = 0;
-    // [0]=after alpha, [1]=after beta; per phase x/y/z
-    CopyOp *d_ops[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-    size_t ops_cap[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    int nops[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    int nblocks[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    bool fuse_x = false;  // x faces exchanged inside the step kernel where the geometry allows
+    // [0]=after alpha, [1]=after beta; per phase x/y/z, and [3] the x-face copies
+    // that the step kernel performs when fused (launched only by lbm_group_sync)
+    CopyOp *d_ops[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
+    size_t ops_cap[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    int nops[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    int nblocks[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    struct XFuse {
+        void *xnb[2];
+        unsigned xmask_a[2];
+        int xrecv[2];
+    };
+    std::vector<XFuse> xfuse;  // per member, from the plan
     long long step_counter = 0;
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
@@ -1235,7 +1243,7 @@ static void group_drop_graph(lbm_group *g) {
 
 static void group_free_plan(lbm_group *g) {
     for (int k = 0; k < 2; ++k)
-        for (int p = 0; p < 3; ++p) {
+        for (int p = 0; p < 4; ++p) {
             if (g->d_ops[k][p]) (void)hipFree(g->d_ops[k][p]);
             g->d_ops[k][p] = nullptr;
             g->ops_cap[k][p] = 0;
@@ -1274,6 +1282,13 @@ static int group_upload_descs(lbm_group *g) {
         lbm_solver *m = g->members[i];
         fill_desc(m);
         DomDev d = m->host_desc;
+        if (i < g->xfuse.size()) {
+            for (int side = 0; side < 2; ++side) {
+                d.xnb[side] = g->xfuse[i].xnb[side];
+                d.xmask_a[side] = g->xfuse[i].xmask_a[side];
+                d.xrecv[side] = g->xfuse[i].xrecv[side];
+            }
+        }
         const int nt = (int)((m->cells + per_tile - 1) / per_tile);
         d.tile_begin = (int)tiles.size();
         d.ntiles = nt;
@@ -1503,14 +1518,42 @@ extern "C" int lbm_group_set_exchange_mode(lbm_group *g, int mode) {
     return LBM_OK;
 }
 
+extern "C" int lbm_group_set_fused_x(lbm_group *g, int enable) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    g->fuse_x = enable != 0;
+    g->plan_built = false;
+    return LBM_OK;
+}
+
 extern "C" int lbm_group_set_graph(lbm_group *g, int enable) {
     if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
     g->use_graph = enable != 0;
     return LBM_OK;
 }
 
+// The copy plan of the exchange.  With fuse_x, an x-face comm between
+// equally sized members with the reference's face geometry is done by the
+// step kernel instead: the sender across the face writes the receiver's
+// ghost layer (after alpha) and inner layer (after beta) directly; its
+// copies are kept for lbm_group_sync only.
+static bool fusable_x(const lbm_group *g, const GroupComm &c, const GroupComm &p) {
+    if (!g->fuse_x || c.normal[0] == 0) return false;
+    const lbm_solver *me = g->members[c.member], *nb = g->members[c.dst];
+    const int X = me->cfg.dims[0], Y = me->cfg.dims[1], Z = me->cfg.dims[2];
+    if (X < 4 || nb->cfg.dims[0] != X || nb->cfg.dims[1] != Y || nb->cfg.dims[2] != Z) return false;
+    if (c.size[0] != 1 || c.size[1] != Y || c.size[2] != Z || p.size[0] != 1 || p.size[1] != Y || p.size[2] != Z)
+        return false;
+    for (int a = 1; a < 3; ++a)
+        if (c.send_origin[a] || c.recv_origin[a] || p.send_origin[a] || p.recv_origin[a]) return false;
+    if (c.normal[0] == 1)  // my x- face: ghost 0, inner 1; the neighbour's x+ face
+        return c.recv_origin[0] == 0 && c.send_origin[0] == 1 && p.send_origin[0] == X - 2 &&
+               p.recv_origin[0] == X - 1;
+    return c.recv_origin[0] == X - 1 && c.send_origin[0] == X - 2 && p.send_origin[0] == 1 && p.recv_origin[0] == 0;
+}
+
 static int build_plan(lbm_group *g) {
-    std::vector<CopyOp> ops[2][3];
+    std::vector<CopyOp> ops[2][4];
+    g->xfuse.assign(g->members.size(), lbm_group::XFuse{{nullptr, nullptr}, {0u, 0u}, {0, 0}});
     for (const GroupComm &c : g->comms) {
         const GroupComm *p = nullptr;
         for (const GroupComm &q : g->comms)
@@ -1533,18 +1576,27 @@ static int build_plan(lbm_group *g) {
             for (int i = 0; i < 19; ++i)
                 if (c.normal[0] * ex(i) + c.normal[1] * ey(i) + c.normal[2] * ez(i) < 0) amask |= 1u << i;
         }
+        const bool fused = fusable_x(g, c, *p);
+        const int slot = fused ? 3 : axis;
+        if (fused) {
+            // the sender is the neighbour, across its face opposite to mine
+            const int side = c.normal[0] == 1 ? 1 : 0;
+            g->xfuse[c.dst].xnb[side] = me->dd;
+            g->xfuse[c.dst].xmask_a[side] = amask;
+            g->xfuse[c.member].xrecv[1 - side] = 1;
+        }
         CopyOp a;
         int rc = fill_op(nb, p->send_origin, me, c.recv_origin, c.size, amask, a, why);
         if (rc) return gfail(g, rc, why);
-        if (a.npop) ops[0][axis].push_back(a);
+        if (a.npop) ops[0][slot].push_back(a);
         // after beta: neighbour's ghost layer -> my inner layer, n.e > 0 only (src/CController.hpp:322-383)
         CopyOp b;
         rc = fill_op(nb, p->recv_origin, me, c.send_origin, c.size, norm_mask(c.normal), b, why);
         if (rc) return gfail(g, rc, why);
-        if (b.npop) ops[1][axis].push_back(b);
+        if (b.npop) ops[1][slot].push_back(b);
     }
     for (int k = 0; k < 2; ++k)
-        for (int p = 0; p < 3; ++p) {
+        for (int p = 0; p < 4; ++p) {
             int blocks = 0;
             for (auto &op : ops[k][p]) {
                 op.block_begin = blocks;
@@ -1559,19 +1611,20 @@ static int build_plan(lbm_group *g) {
         }
     GCHK(hipStreamSynchronize(g->stream));
     g->plan_built = true;
+    g->descs_dirty = true;  // the descriptors carry the fused faces
     return LBM_OK;
 }
 
 // Brings descriptors and the exchange plan up to date before launches.
 static int group_prepare(lbm_group *g) {
+    if (!g->plan_built) {
+        int rc = build_plan(g);
+        if (rc) return rc;
+    }
     bool stale = g->descs_dirty;
     for (size_t i = 0; i < g->members.size(); ++i) stale |= g->seen_versions[i] != g->members[i]->version;
     if (stale) {
         int rc = group_upload_descs(g);
-        if (rc) return rc;
-    }
-    if (!g->plan_built) {
-        int rc = build_plan(g);
         if (rc) return rc;
     }
     return LBM_OK;
@@ -1580,7 +1633,7 @@ static int group_prepare(lbm_group *g) {
 static std::vector<int> group_shape(const lbm_group *g) {
     std::vector<int> v = {g->ntiles, g->wg, g->variant, g->lds};
     for (int k = 0; k < 2; ++k)
-        for (int p = 0; p < 3; ++p) {
+        for (int p = 0; p < 4; ++p) {
             v.push_back(g->nops[k][p]);
             v.push_back(g->nblocks[k][p]);
         }
@@ -1765,7 +1818,10 @@ extern "C" int lbm_group_sync(lbm_group *g, int after_beta) {
         int rc = group_prepare(g);
         if (rc) return rc;
     }
-    for (int p = 0; p < 3; ++p) {
+    // fused x copies first: they belong to the x phase, disjoint from the rest of it
+    const int order[4] = {3, 0, 1, 2};
+    for (int q = 0; q < 4; ++q) {
+        const int p = order[q];
         if (!g->nops[after_beta][p]) continue;
         GCHK(exchange_launch(g->storage, g->d_ops[after_beta][p], g->nops[after_beta][p], g->nblocks[after_beta][p],
                              g->stream));

@@ -200,6 +200,20 @@ class LbmSolver:
         self._check(self._lib.lbm_solver_mass_energy(self._h, C.byref(m), C.byref(k)))
         return m.value, k.value
 
+    def checkpoint(self) -> dict:
+        """Full state for resume: distributions in the compute type, flags and
+        the step counter (the parity the distributions are stored in)."""
+        return {"dd": self.store(_capi.FIELD_DD), "flags": self.store(_capi.FIELD_FLAGS),
+                "step_counter": self.step_counter, "storage": self.storage, "dims": self.dims}
+
+    def restore(self, state: dict):
+        """Load a checkpoint taken from a solver of the same dims and storage."""
+        if tuple(state["dims"]) != self.dims or int(state["storage"]) != self.storage:
+            raise ValueError("checkpoint of a different domain or storage")
+        self.set(_capi.FIELD_FLAGS, state["flags"])
+        self.set(_capi.FIELD_DD, state["dd"])
+        self.step_counter = int(state["step_counter"])
+
     def moments(self, origin=None, size=None):
         o, s, shape = self._box(origin, size)
         rho = np.empty((shape[2], shape[1], shape[0]), dtype=self.dtype)
@@ -281,6 +295,10 @@ class LbmGroup:
 
     def set_exchange_mode(self, mode: int):
         self._check(self._lib.lbm_group_set_exchange_mode(self._h, int(mode)))
+
+    def set_fused_x(self, enable: bool):
+        """Exchange same-sized x faces inside the step kernel."""
+        self._check(self._lib.lbm_group_set_fused_x(self._h, int(bool(enable))))
 
     def set_graph(self, enable: bool):
         self._check(self._lib.lbm_group_set_graph(self._h, int(bool(enable))))
