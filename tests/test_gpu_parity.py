@@ -338,3 +338,43 @@ def test_fused_x_periodic_channel():
         g.close()
         s.close()
     np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
+@pytest.mark.parametrize("dims", [(20, 18, 16), (21, 17, 19), (64, 12, 10)])
+def test_two_byte_storage_launch_shapes_identical(storage, dims):
+    """FP16/BF16 storage: 1, 2 and 4 cells per lane (vector accesses, element
+    accesses where a shifted run straddles a layout block), work-groups
+    64/128/256, give bit-identical states."""
+    say(f"case: two-byte storage {storage} dims {dims} launch shapes")
+    out = []
+    for variant, wg in ((0, 256), (8, 256), (8, 64), (16, 128)):
+        s = LbmSolver(dims, (1,) * 6, inv_tau=1 / 0.53, tau=0.53, smagorinsky_cs=0.17, lid_velocity=0.06,
+                      storage=storage, kernel_variant=variant, work_group_size=wg)
+        s.init_perturbed(1e-3, 17)
+        o, sz = dec.lid_region(dims)
+        s.set(_capi.FIELD_FLAGS, np.full(sz[::-1], orc.FLAG_INJECTION, np.int32), o, sz)
+        s.run(9)
+        out.append(gpu_dd(s))
+        s.close()
+    for other in out[1:]:
+        np.testing.assert_array_equal(out[0], other)
+
+
+def test_two_byte_channel_pairs_with_fused_x():
+    """Vector lanes next to fused periodic x faces (edge lanes take the
+    element path): FP16 channel, 2 cells per lane == 1 cell per lane."""
+    say("case: fp16 channel pairs + fused x")
+    from turbulent_lbm_multigpu_amd import precision
+
+    out = []
+    for variant in (0, 8):
+        s, g, _ = precision.build_channel((32, 18, 16), _capi.STORAGE_F16_SHIFTED, tau=0.55, gx=2e-5, fill=0.04,
+                                          rmin=2.0, rmax=3.0, amp=1e-3, seed=9, kernel_variant=variant)
+        g.set_fused_x(True)
+        g.step(15)
+        g.wait()
+        out.append(gpu_dd(s))
+        g.close()
+        s.close()
+    np.testing.assert_array_equal(out[0], out[1])
