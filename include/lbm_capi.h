@@ -60,6 +60,8 @@ extern "C" {
 #define LBM_FIELD_VELOCITY 1 /* 3 components, compute type */
 #define LBM_FIELD_DENSITY 2  /* 1 component, compute type */
 #define LBM_FIELD_FLAGS 3    /* 1 component, int32 */
+#define LBM_FIELD_DD_RAW 4   /* 19 components, the stored bit patterns (f32/f64, or f - w as f16/bf16):
+                                an exact checkpoint of any storage (new) */
 
 typedef struct lbm_solver lbm_solver;
 typedef struct lbm_group lbm_group;

@@ -48,3 +48,21 @@ def test_checkpoint_resume_bitwise(storage, first):
     b.run(total - first)
     np.testing.assert_array_equal(b.store(_capi.FIELD_DD), want)
     b.close()
+
+
+@pytest.mark.parametrize("storage", [_capi.STORAGE_F32, _capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
+def test_raw_field_roundtrip_and_region(storage):
+    """LBM_FIELD_DD_RAW moves the stored bits: a region set from a store is a
+    no-op, and the raw bits of FP32 storage are the FP32 field's bits."""
+    say(f"case: raw field storage {storage}")
+    s = cavity(storage)
+    s.run(5)
+    raw = s.store(_capi.FIELD_DD_RAW)
+    o, sz = (3, 2, 4), (7, 5, 6)
+    part = s.store(_capi.FIELD_DD_RAW, o, sz)
+    np.testing.assert_array_equal(part, raw[:, 4:10, 2:7, 3:10])
+    s.set(_capi.FIELD_DD_RAW, part, o, sz)
+    np.testing.assert_array_equal(s.store(_capi.FIELD_DD_RAW), raw)
+    if storage == _capi.STORAGE_F32:
+        np.testing.assert_array_equal(raw.view(np.float32), s.store(_capi.FIELD_DD))
+    s.close()

@@ -19,15 +19,14 @@ from turbulent_lbm_multigpu_amd.solver import LbmSolver  # noqa: E402
 
 # Stated bounds for the reduced channel (96x48x48, 2000 steps): rho as an
 # absolute error, u relative to the largest FP64 speed U over fluid cells, ke
-# relative.  Measured on MI355X (deterministic): f32 rho 6.4e-5 / u 5.8e-3 /
-# ke 1.1e-2, f16 8.1e-4 / 5.3e-3 / 9.9e-3, bf16 6.9e-4 / 1.0e-2 / 7.5e-3.  The
-# FP32 collision dominates: the per-step body-force increment 3 w g ~ 1.7e-7 is
-# only ~45 ulp of an FP32 population ~0.055, so every storage that collides in
-# FP32 drifts from FP64 by ~1% in kinetic energy after 2000 steps; the shifted
-# FP16/BF16 storage adds little on top (u_rms within 1.5x of FP32's).
+# relative.  FP32 storage runs the reference's arithmetic on f, where the
+# per-step body-force increment 3 w g ~ 1.7e-7 is only ~45 ulp of a population
+# ~0.055: it drifts from FP64 by ~1% in kinetic energy after 2000 steps.  The
+# shifted storages collide in f - w, which keeps those digits: FP16 ends far
+# closer to FP64 than FP32 does; BF16 is limited by its 8-bit significand.
 BOUNDS = {
     "f32": dict(rho=2e-4, u=1.5e-2, ke=3e-2),
-    "f16": dict(rho=2.5e-3, u=1.5e-2, ke=3e-2),
+    "f16": dict(rho=5e-4, u=1e-2, ke=5e-3),
     "bf16": dict(rho=2.5e-3, u=3e-2, ke=3e-2),
 }
 RMS_VS_F32 = 1.5
@@ -68,5 +67,9 @@ def test_channel_storage_errors_vs_fp64():
         assert r["u_max_abs_err_over_U"] < b["u"], (name, r)
         assert r["ke_rel_err"] < b["ke"], (name, r)
         assert r["u_rms_err_over_U"] <= RMS_VS_F32 * res["f32"]["u_rms_err_over_U"], (name, r)
+    # FP16 storage with the shifted collision beats FP32 storage against FP64
+    assert res["f16"]["u_rms_err_over_U"] < res["f32"]["u_rms_err_over_U"], res
+    assert res["f16"]["ke_rel_err"] < res["f32"]["ke_rel_err"], res
+    assert res["f16"]["mass_rel_err"] < res["f32"]["mass_rel_err"], res
     # the flow is driven: the largest speed is far above the initial noise
     assert res["f32"]["U_ref"] > 1e-4

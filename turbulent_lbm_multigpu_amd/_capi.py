@@ -39,6 +39,7 @@ FIELD_DD = 0
 FIELD_VELOCITY = 1
 FIELD_DENSITY = 2
 FIELD_FLAGS = 3
+FIELD_DD_RAW = 4
 
 CASE_CAVITY = 0
 CASE_CHANNEL = 1

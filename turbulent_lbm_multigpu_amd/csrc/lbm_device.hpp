@@ -120,38 +120,64 @@ __device__ __forceinline__ void raw_st(gchar *base, unsigned off, U v) {
 // to_bits convert one element of population slot i.
 template <int ST>
 struct Storage;
+// kShifted storages hold f - w_i: from_bits/to_bits convert to and from f,
+// from_raw/to_raw keep the shifted value (the step kernel collides in it).
 template <>
 struct Storage<0> {
     using U = unsigned;
     using T = float;
     static constexpr unsigned kBytes = 4;
+    static constexpr bool kShifted = false;
     __device__ static T from_bits(U u, int) { return __uint_as_float(u); }
     __device__ static U to_bits(T v, int) { return __float_as_uint(v); }
+    __device__ static T from_raw(U u) { return __uint_as_float(u); }
+    __device__ static U to_raw(T v) { return __float_as_uint(v); }
 };
 template <>
 struct Storage<1> {
     using U = unsigned long long;
     using T = double;
     static constexpr unsigned kBytes = 8;
+    static constexpr bool kShifted = false;
     __device__ static T from_bits(U u, int) { return __longlong_as_double((long long)u); }
     __device__ static U to_bits(T v, int) { return (U)__double_as_longlong(v); }
+    __device__ static T from_raw(U u) { return __longlong_as_double((long long)u); }
+    __device__ static U to_raw(T v) { return (U)__double_as_longlong(v); }
 };
 template <>
 struct Storage<2> {
     using U = unsigned short;
     using T = float;
     static constexpr unsigned kBytes = 2;
-    __device__ static T from_bits(U u, int i) { return __half2float(__ushort_as_half(u)) + weight<float>(i); }
-    __device__ static U to_bits(T v, int i) { return __half_as_ushort(__float2half(v - weight<float>(i))); }
+    static constexpr bool kShifted = true;
+    __device__ static T from_raw(U u) { return __half2float(__ushort_as_half(u)); }
+    __device__ static U to_raw(T v) { return __half_as_ushort(__float2half(v)); }
+    __device__ static T from_bits(U u, int i) { return from_raw(u) + weight<float>(i); }
+    __device__ static U to_bits(T v, int i) { return to_raw(v - weight<float>(i)); }
 };
 template <>
 struct Storage<3> {
     using U = unsigned short;
     using T = float;
     static constexpr unsigned kBytes = 2;
-    __device__ static T from_bits(U u, int i) { return __uint_as_float((unsigned)u << 16) + weight<float>(i); }
-    __device__ static U to_bits(T v, int i) { return bf16_bits_rne(v - weight<float>(i)); }
+    static constexpr bool kShifted = true;
+    __device__ static T from_raw(U u) { return __uint_as_float((unsigned)u << 16); }
+    __device__ static U to_raw(T v) { return bf16_bits_rne(v); }
+    __device__ static T from_bits(U u, int i) { return from_raw(u) + weight<float>(i); }
+    __device__ static U to_bits(T v, int i) { return to_raw(v - weight<float>(i)); }
 };
+
+// Conversions of the step kernel: shifted storages stay shifted in registers.
+template <class SP>
+__device__ __forceinline__ typename SP::T kin(typename SP::U u, int i) {
+    if (SP::kShifted) return SP::from_raw(u);
+    return SP::from_bits(u, i);
+}
+template <class SP>
+__device__ __forceinline__ typename SP::U kout(typename SP::T v, int i) {
+    if (SP::kShifted) return SP::to_raw(v);
+    return SP::to_bits(v, i);
+}
 
 // one element of population slot i at byte offset `off` from its population base
 template <class SP, bool NT = false>
@@ -163,21 +189,32 @@ __device__ __forceinline__ void st(gchar *base, unsigned off, int i, typename SP
     raw_st<NT, typename SP::U>(base, off, SP::to_bits(v, i));
 }
 
-// N consecutive elements as one vector access (alignment: one element).
+// the step kernel's element access (kin/kout conversions)
+template <class SP, bool NT = false>
+__device__ __forceinline__ typename SP::T kld(const gchar *base, unsigned off, int i) {
+    return kin<SP>(raw_ld<NT, typename SP::U>(base, off), i);
+}
+template <class SP, bool NT = false>
+__device__ __forceinline__ void kst(gchar *base, unsigned off, int i, typename SP::T v) {
+    raw_st<NT, typename SP::U>(base, off, kout<SP>(v, i));
+}
+
+// N consecutive elements as one vector access (alignment: one element),
+// with the step kernel's conversions.
 template <class SP, int N>
 __device__ __forceinline__ void vld(const gchar *base, unsigned off, int i, typename SP::T *out) {
     typedef typename SP::U V __attribute__((ext_vector_type(N)));
     V v;
     __builtin_memcpy(&v, base + off, sizeof(V));
 #pragma unroll
-    for (int j = 0; j < N; ++j) out[j] = SP::from_bits(v[j], i);
+    for (int j = 0; j < N; ++j) out[j] = kin<SP>(v[j], i);
 }
 template <class SP, int N>
 __device__ __forceinline__ void vst(gchar *base, unsigned off, int i, const typename SP::T *in) {
     typedef typename SP::U V __attribute__((ext_vector_type(N)));
     V v;
 #pragma unroll
-    for (int j = 0; j < N; ++j) v[j] = SP::to_bits(in[j], i);
+    for (int j = 0; j < N; ++j) v[j] = kout<SP>(in[j], i);
     __builtin_memcpy(base + off, &v, sizeof(V));
 }
 
@@ -271,8 +308,10 @@ __device__ __forceinline__ T vel2_of(T vx, T vy, T vz) {
 // tau_eff = (tau0 + sqrt(tau0^2 + les_coeff*|Pi_neq|/rho)) / 2,
 // les_coeff = 18 sqrt(2) Cs^2, Pi_neq_ab = sum e_a e_b f - rho/3 d_ab - j_a j_b.
 // The 19 populations are in this work-item's registers: no cross-lane traffic.
+// rho_m: the density as summed from f[] (rho, or rho - 1 for shifted f),
+// rho: the full density.
 template <typename T>
-__device__ __forceinline__ T les_inv_tau(const T *f, T rho, T jx, T jy, T jz, T tau0, T les_coeff) {
+__device__ __forceinline__ T les_inv_tau(const T *f, T rho_m, T rho, T jx, T jy, T jz, T tau0, T les_coeff) {
     T pxx = f[0] + f[1];
     pxx = pxx + f[4]; pxx = pxx + f[5]; pxx = pxx + f[6]; pxx = pxx + f[7];
     pxx = pxx + f[8]; pxx = pxx + f[9]; pxx = pxx + f[10]; pxx = pxx + f[11];
@@ -285,7 +324,7 @@ __device__ __forceinline__ T les_inv_tau(const T *f, T rho, T jx, T jy, T jz, T 
     T pxy = (f[4] + f[5]) - (f[6] + f[7]);
     T pxz = (f[8] + f[9]) - (f[10] + f[11]);
     T pyz = (f[12] + f[13]) - (f[14] + f[15]);
-    T r3 = rho * LC<T>::w0;
+    T r3 = rho_m * LC<T>::w0;
     T qxx = (pxx - r3) - jx * jx;
     T qyy = (pyy - r3) - jy * jy;
     T qzz = (pzz - r3) - jz * jz;
@@ -340,10 +379,10 @@ __device__ __forceinline__ T alpha_ref_forcing(T *f, T rho, T gx, T gy, T gz) {
 // FLUID collision shared by both kernels: optional LES rate, then BGK
 // f_i += w (feq_i - f_i) (src/cl_programs/lbm_alpha.cl:185-300).
 template <typename T>
-__device__ __forceinline__ void collide_fluid(T *f, T rho, T vx, T vy, T vz, T dd_param, T inv_tau, T tau0,
-                                              T les_coeff) {
+__device__ __forceinline__ void collide_fluid(T *f, T rho, T rho_full, T vx, T vy, T vz, T dd_param, T inv_tau,
+                                              T tau0, T les_coeff) {
     T w = inv_tau;
-    if (les_coeff > T(0)) w = les_inv_tau(f, rho, vx, vy, vz, tau0, les_coeff);
+    if (les_coeff > T(0)) w = les_inv_tau(f, rho, rho_full, vx, vy, vz, tau0, les_coeff);
     T q[19];
     feq19(dd_param, vx, vy, vz, q);
 #pragma unroll
@@ -372,7 +411,13 @@ __device__ __forceinline__ int wrap_idx(int a, int cells) {
 // of one cell.  f[] holds the gathered (beta) or local (alpha) populations on
 // entry and the values to store on exit; returns whether they are stored
 // (alpha stores FLUID and VELOCITY_INJECTION cells only, beta every cell).
-template <typename T, bool BETA>
+//
+// SHIFT: f[] holds f_i - w_i (shifted storage).  Then `rho` below sums to
+// rho - 1, j is unchanged (sum w_i e_i = 0), the equilibrium from
+// dd_param - 1 is f_eq - w, and the Smagorinsky stress loses the w terms
+// exactly (sum e_a e_b w_i = delta_ab / 3 cancels rho / 3); the full density
+// enters only the LES divisor, the forcing and the density output.
+template <typename T, bool BETA, bool SHIFT = false>
 __device__ __forceinline__ bool cell_update(T *f, int flag, const DomDev *D, T &vx, T &vy, T &vz, T &dens_out) {
     T rho;
     if (BETA) {  // summation order of lbm_beta.cl:268-483
@@ -416,17 +461,18 @@ __device__ __forceinline__ bool cell_update(T *f, int flag, const DomDev *D, T &
         rho = rho + f[17]; vz = vz - f[17];
         rho = rho + f[18];
     }
-    dens_out = rho;
+    const T rho_full = SHIFT ? T(1) + rho : rho;
+    dens_out = rho_full;
     bool write_dd = BETA;
     if (flag == kFlagFluid) {
         const T dd_param = rho - T(1.5) * vel2_of(vx, vy, vz);
-        collide_fluid(f, rho, vx, vy, vz, dd_param, (T)D->inv_tau, (T)D->tau0, (T)D->les_coeff);
+        collide_fluid(f, rho, rho_full, vx, vy, vz, dd_param, (T)D->inv_tau, (T)D->tau0, (T)D->les_coeff);
         if (D->forcing == 1) {
-            body_force(f, rho, (T)D->gx, (T)D->gy, (T)D->gz);
+            body_force(f, rho_full, (T)D->gx, (T)D->gy, (T)D->gz);
         } else if (!BETA) {
-            dens_out = alpha_ref_forcing(f, rho, (T)D->gx, (T)D->gy, (T)D->gz);
+            dens_out = alpha_ref_forcing(f, rho_full, (T)D->gx, (T)D->gy, (T)D->gz);
         } else {
-            dens_out = dd_param;  // `#define dd_param rho` (lbm_beta.cl:493)
+            dens_out = SHIFT ? T(1) + dd_param : dd_param;  // `#define dd_param rho` (lbm_beta.cl:493)
         }
         write_dd = true;
     } else if (flag == kFlagObstacle) {
@@ -442,9 +488,9 @@ __device__ __forceinline__ bool cell_update(T *f, int flag, const DomDev *D, T &
     } else if (flag == kFlagInjection) {  // lbm_alpha.cl:345-480, lbm_beta.cl:582-653
         vx = (T)D->lid; vy = T(0); vz = T(0);
         const T one = T(1);
-        const T dd_param = one - T(1.5) * vel2_of(vx, vy, vz);
+        const T dd_param = SHIFT ? T(-1.5) * vel2_of(vx, vy, vz) : one - T(1.5) * vel2_of(vx, vy, vz);
         feq19(dd_param, vx, vy, vz, f);
-        if (BETA) dens_out = dd_param;
+        if (BETA) dens_out = SHIFT ? one + dd_param : dd_param;
         else if (D->forcing == 0) dens_out = alpha_ref_forcing(f, one, (T)D->gx, (T)D->gy, (T)D->gz);
         else dens_out = one;
         write_dd = true;
@@ -479,17 +525,17 @@ __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int ce
 #pragma unroll
         for (int i = 0; i < 18; ++i) {
             const unsigned idx = wrap ? (unsigned)wrap_idx((int)g - d[i], cells) : g - (unsigned)d[i];
-            f[i] = ld<SP>(pl[opp(i)], cell_base(idx) * SP::kBytes, opp(i));
+            f[i] = kld<SP>(pl[opp(i)], cell_base(idx) * SP::kBytes, opp(i));
         }
-        f[18] = ld<SP>(pl[18], own, 18);
+        f[18] = kld<SP>(pl[18], own, 18);
     } else {
         // the 19 loads do not wait for the flag; a GHOST cell (lbm_alpha.cl:30-32)
         // matches no branch of cell_update and stores nothing
 #pragma unroll
-        for (int i = 0; i < 19; ++i) f[i] = ld<SP>(pl[i], own, i);
+        for (int i = 0; i < 19; ++i) f[i] = kld<SP>(pl[i], own, i);
     }
     T vx, vy, vz, dens;
-    const bool write_dd = cell_update<T, BETA>(f, flag, D, vx, vy, vz, dens);
+    const bool write_dd = cell_update<T, BETA, SP::kShifted>(f, flag, D, vx, vy, vz, dens);
     const int X = D->nx;
     if (write_dd) {
         if (BETA) {
@@ -501,18 +547,18 @@ __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int ce
 #pragma unroll
             for (int i = 0; i < 18; ++i) {
                 const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
-                if (!(ex(i) != 0 && skip_x)) st<SP, NTS>(pl[i], cell_base(idx) * SP::kBytes, i, f[i]);
+                if (!(ex(i) != 0 && skip_x)) kst<SP, NTS>(pl[i], cell_base(idx) * SP::kBytes, i, f[i]);
                 if (ex(i) == 1 && x == X - 2 && D->xnb[1])
-                    st<SP>((gchar *)D->xnb[1] + (size_t)i * kBlk * SP::kBytes, cell_base(idx - X + 2) * SP::kBytes,
+                    kst<SP>((gchar *)D->xnb[1] + (size_t)i * kBlk * SP::kBytes, cell_base(idx - X + 2) * SP::kBytes,
                            i, f[i]);
                 if (ex(i) == -1 && x == 1 && D->xnb[0])
-                    st<SP>((gchar *)D->xnb[0] + (size_t)i * kBlk * SP::kBytes, cell_base(idx + X - 2) * SP::kBytes,
+                    kst<SP>((gchar *)D->xnb[0] + (size_t)i * kBlk * SP::kBytes, cell_base(idx + X - 2) * SP::kBytes,
                            i, f[i]);
             }
-            st<SP, NTS>(pl[18], own, 18, f[18]);
+            kst<SP, NTS>(pl[18], own, 18, f[18]);
         } else {
 #pragma unroll
-            for (int i = 0; i < 19; ++i) st<SP, NTS>(pl[opp(i)], own, opp(i), f[i]);
+            for (int i = 0; i < 19; ++i) kst<SP, NTS>(pl[opp(i)], own, opp(i), f[i]);
         }
     }
     if (!BETA && x >= 0) {
@@ -527,7 +573,7 @@ __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int ce
 #pragma unroll
             for (int k = 0; k < 19; ++k)
                 if (m & (1u << k))
-                    st<SP>(rb + (size_t)k * kBlk * SP::kBytes, cell_base(r) * SP::kBytes, k,
+                    kst<SP>(rb + (size_t)k * kBlk * SP::kBytes, cell_base(r) * SP::kBytes, k,
                            write_dd ? f[opp(k)] : f[k]);
         }
     }
@@ -543,7 +589,7 @@ __device__ __forceinline__ void vld_any(const gchar *p, unsigned s, int i, typen
         vld<SP, N>(p, cell_base(s) * SP::kBytes, i, out);
     } else {
 #pragma unroll
-        for (int j = 0; j < N; ++j) out[j] = ld<SP>(p, cell_base(s + j) * SP::kBytes, i);
+        for (int j = 0; j < N; ++j) out[j] = kld<SP>(p, cell_base(s + j) * SP::kBytes, i);
     }
 }
 template <class SP, int N>
@@ -552,7 +598,7 @@ __device__ __forceinline__ void vst_any(gchar *p, unsigned s, int i, const typen
         vst<SP, N>(p, cell_base(s) * SP::kBytes, i, in);
     } else {
 #pragma unroll
-        for (int j = 0; j < N; ++j) st<SP>(p, cell_base(s + j) * SP::kBytes, i, in[j]);
+        for (int j = 0; j < N; ++j) kst<SP>(p, cell_base(s + j) * SP::kBytes, i, in[j]);
     }
 }
 
@@ -588,7 +634,7 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
         T f[19];
 #pragma unroll
         for (int i = 0; i < 19; ++i) f[i] = F[i][j];
-        const bool w = cell_update<T, BETA>(f, flag[j], D, vx[j], vy[j], vz[j], dn[j]);
+        const bool w = cell_update<T, BETA, SP::kShifted>(f, flag[j], D, vx[j], vy[j], vz[j], dn[j]);
         all_write = all_write && w;
         any_ghost = any_ghost || flag[j] == kFlagGhost;
 #pragma unroll
@@ -607,7 +653,7 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
         for (int j = 0; j < N; ++j)
             if (flag[j] >= 0)
 #pragma unroll
-                for (int i = 0; i < 19; ++i) st<SP>(pl[opp(i)], own + j * es, opp(i), F[i][j]);
+                for (int i = 0; i < 19; ++i) kst<SP>(pl[opp(i)], own + j * es, opp(i), F[i][j]);
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {

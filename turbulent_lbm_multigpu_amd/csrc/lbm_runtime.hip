@@ -124,7 +124,12 @@ __global__ void pack_kernel(DomDev D, int field, Box b, void *out) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const long long g = box_cell(D, b, e);
-    if (field == LBM_FIELD_DD) {
+    if (field == LBM_FIELD_DD_RAW) {  // stored bit patterns, no conversion
+        using U = typename SP::U;
+        U *o = (U *)out;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) o[i * n + e] = raw_ld<false, U>(pop_ptr<SP>(D, i), cell_off<SP>(g));
+    } else if (field == LBM_FIELD_DD) {
         T *o = (T *)out;
 #pragma unroll
         for (int i = 0; i < 19; ++i) o[i * n + e] = ld<SP>(pop_ptr<SP>(D, i), cell_off<SP>(g), i);
@@ -148,7 +153,13 @@ __global__ void unpack_kernel(DomDev D, int field, Box b, const void *in, uint32
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const long long g = box_cell(D, b, e);
-    if (field == LBM_FIELD_DD) {
+    if (field == LBM_FIELD_DD_RAW) {
+        using U = typename SP::U;
+        const U *s = (const U *)in;
+#pragma unroll
+        for (int i = 0; i < 19; ++i)
+            if (mask & (1u << i)) raw_st<false, U>(pop_ptr<SP>(D, i), cell_off<SP>(g), s[i * n + e]);
+    } else if (field == LBM_FIELD_DD) {
         const T *s = (const T *)in;
 #pragma unroll
         for (int i = 0; i < 19; ++i)
@@ -532,7 +543,14 @@ int check_box(lbm_solver *s, const int *origin, const int *size, Box &b) {
 }
 
 int field_components(int field) {
-    return field == LBM_FIELD_DD ? 19 : field == LBM_FIELD_VELOCITY ? 3 : 1;
+    return (field == LBM_FIELD_DD || field == LBM_FIELD_DD_RAW) ? 19 : field == LBM_FIELD_VELOCITY ? 3 : 1;
+}
+
+// bytes of one component value of a region transfer
+size_t field_elem_bytes(const lbm_solver *s, int field) {
+    if (field == LBM_FIELD_FLAGS) return 4;
+    if (field == LBM_FIELD_DD_RAW) return elem_size(s->cfg.storage);
+    return compute_size(s->cfg.storage);
 }
 
 }  // namespace
@@ -858,7 +876,7 @@ extern "C" int lbm_solver_set_relaxation(lbm_solver *s, double inv_tau, double t
 
 extern "C" int lbm_solver_store(lbm_solver *s, int field, const int origin[3], const int size[3], void *host_dst) {
     if (!s || !host_dst || (origin && !size)) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
-    if (field < 0 || field > 3) return fail(s, LBM_ERR_INVALID_ARGUMENT, "unknown field");
+    if (field < 0 || field > 4) return fail(s, LBM_ERR_INVALID_ARGUMENT, "unknown field");
     if (field == LBM_FIELD_VELOCITY && !s->vel)
         return fail(s, LBM_ERR_STATE, "velocity is not stored by this solver (store_velocity = 0)");
     if (field == LBM_FIELD_DENSITY && !s->dens)
@@ -867,7 +885,7 @@ extern "C" int lbm_solver_store(lbm_solver *s, int field, const int origin[3], c
     int rc = check_box(s, origin, size, b);
     if (rc) return rc;
     const long long n = (long long)b.s[0] * b.s[1] * b.s[2];
-    const size_t bytes = (size_t)n * field_components(field) * (field == LBM_FIELD_FLAGS ? 4 : compute_size(s->cfg.storage));
+    const size_t bytes = (size_t)n * field_components(field) * field_elem_bytes(s, field);
     rc = ensure_staging(s, bytes);
     if (rc) return rc;
     rc = dispatch_storage(s->cfg.storage, [&](auto st) -> int {
@@ -885,8 +903,9 @@ extern "C" int lbm_solver_store(lbm_solver *s, int field, const int origin[3], c
 extern "C" int lbm_solver_set(lbm_solver *s, int field, const int origin[3], const int size[3], const void *host_src,
                               const int norm[3]) {
     if (!s || !host_src || (origin && !size)) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
-    if (field < 0 || field > 3) return fail(s, LBM_ERR_INVALID_ARGUMENT, "unknown field");
-    if (norm && field != LBM_FIELD_DD) return fail(s, LBM_ERR_INVALID_ARGUMENT, "norm filter applies to distributions only");
+    if (field < 0 || field > 4) return fail(s, LBM_ERR_INVALID_ARGUMENT, "unknown field");
+    if (norm && field != LBM_FIELD_DD && field != LBM_FIELD_DD_RAW)
+        return fail(s, LBM_ERR_INVALID_ARGUMENT, "norm filter applies to distributions only");
     if (field == LBM_FIELD_VELOCITY && !s->vel)
         return fail(s, LBM_ERR_STATE, "velocity is not stored by this solver (store_velocity = 0)");
     if (field == LBM_FIELD_DENSITY && !s->dens)
@@ -901,7 +920,7 @@ extern "C" int lbm_solver_set(lbm_solver *s, int field, const int origin[3], con
             if (fl[i] < 0 || fl[i] > 255)
                 return fail(s, LBM_ERR_INVALID_ARGUMENT, "flag values must be in [0, 255]");
     }
-    const size_t bytes = (size_t)n * field_components(field) * (field == LBM_FIELD_FLAGS ? 4 : compute_size(s->cfg.storage));
+    const size_t bytes = (size_t)n * field_components(field) * field_elem_bytes(s, field);
     rc = ensure_staging(s, bytes);
     if (rc) return rc;
     SCHK(hipMemcpyAsync(s->staging, host_src, bytes, hipMemcpyHostToDevice, s->stream));

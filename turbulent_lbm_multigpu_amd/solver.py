@@ -24,7 +24,10 @@ D3Q19_E = np.array(
      [0, 0, 1], [0, 0, -1], [0, 0, 0]], dtype=np.int64)
 """D3Q19 directions in the reference's order (src/main.cpp:38-66)."""
 
-_COMPONENTS = {_capi.FIELD_DD: 19, _capi.FIELD_VELOCITY: 3, _capi.FIELD_DENSITY: 1, _capi.FIELD_FLAGS: 1}
+_COMPONENTS = {_capi.FIELD_DD: 19, _capi.FIELD_VELOCITY: 3, _capi.FIELD_DENSITY: 1, _capi.FIELD_FLAGS: 1,
+               _capi.FIELD_DD_RAW: 19}
+_RAW_DTYPE = {_capi.STORAGE_F32: np.uint32, _capi.STORAGE_F64: np.uint64, _capi.STORAGE_F16_SHIFTED: np.uint16,
+              _capi.STORAGE_BF16_SHIFTED: np.uint16}
 
 
 def compute_dtype(storage: int):
@@ -158,6 +161,13 @@ class LbmSolver:
         self._check(self._lib.lbm_solver_set_relaxation(self._h, float(inv_tau), float(tau), float(smagorinsky_cs)))
 
     # -- region transfer (src/CLbmSolver.hpp:688-978)
+    def _field_dtype(self, field: int):
+        if field == _capi.FIELD_FLAGS:
+            return np.int32
+        if field == _capi.FIELD_DD_RAW:
+            return _RAW_DTYPE[self.storage]
+        return self.dtype
+
     def _box(self, origin, size):
         if origin is None:
             return None, None, tuple(self.dims)
@@ -166,7 +176,7 @@ class LbmSolver:
     def store(self, field: int, origin=None, size=None) -> np.ndarray:
         o, s, shape = self._box(origin, size)
         ncomp = _COMPONENTS[field]
-        dt = np.int32 if field == _capi.FIELD_FLAGS else self.dtype
+        dt = self._field_dtype(field)
         out = np.empty((ncomp, shape[2], shape[1], shape[0]), dtype=dt)
         self._check(self._lib.lbm_solver_store(self._h, int(field), o, s, out.ctypes.data_as(C.c_void_p)))
         return out if ncomp > 1 else out[0]
@@ -174,7 +184,7 @@ class LbmSolver:
     def set(self, field: int, data, origin=None, size=None, norm=None):
         o, s, shape = self._box(origin, size)
         ncomp = _COMPONENTS[field]
-        dt = np.int32 if field == _capi.FIELD_FLAGS else self.dtype
+        dt = self._field_dtype(field)
         arr = np.ascontiguousarray(np.asarray(data, dtype=dt)).reshape(ncomp, shape[2], shape[1], shape[0])
         self._check(self._lib.lbm_solver_set(self._h, int(field), o, s, arr.ctypes.data_as(C.c_void_p),
                                              ivec(norm)))
@@ -201,9 +211,10 @@ class LbmSolver:
         return m.value, k.value
 
     def checkpoint(self) -> dict:
-        """Full state for resume: distributions in the compute type, flags and
-        the step counter (the parity the distributions are stored in)."""
-        return {"dd": self.store(_capi.FIELD_DD), "flags": self.store(_capi.FIELD_FLAGS),
+        """Full state for resume: the distributions' stored bit patterns (exact
+        for every storage), flags and the step counter (the parity the
+        distributions are stored in)."""
+        return {"dd_raw": self.store(_capi.FIELD_DD_RAW), "flags": self.store(_capi.FIELD_FLAGS),
                 "step_counter": self.step_counter, "storage": self.storage, "dims": self.dims}
 
     def restore(self, state: dict):
@@ -211,7 +222,7 @@ class LbmSolver:
         if tuple(state["dims"]) != self.dims or int(state["storage"]) != self.storage:
             raise ValueError("checkpoint of a different domain or storage")
         self.set(_capi.FIELD_FLAGS, state["flags"])
-        self.set(_capi.FIELD_DD, state["dd"])
+        self.set(_capi.FIELD_DD_RAW, state["dd_raw"])
         self.step_counter = int(state["step_counter"])
 
     def moments(self, origin=None, size=None):
