@@ -256,6 +256,7 @@ typedef struct lbm_server lbm_server;
 #define LBM_CASE_QUEUED 0
 #define LBM_CASE_RUNNING 1
 #define LBM_CASE_DONE 2
+#define LBM_CASE_FAILED 3 /* setup failed: lbm_server_fetch reports why and releases it */
 
 typedef struct lbm_server_config {
     int device;
@@ -293,7 +294,7 @@ typedef struct lbm_case_status {
 } lbm_case_status;
 
 typedef struct lbm_server_stats {
-    long long submitted, completed, chunks, steps_cells;  /* steps_cells: sum of cells x steps run */
+    long long submitted, completed, failed, chunks, steps_cells;  /* steps_cells: sum of cells x steps run */
     size_t bytes_in_use, bytes_peak;
     int queued, active, ready;
     double device_ms;  /* sum over chunks of the device time of the fused launches */

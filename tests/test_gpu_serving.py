@@ -99,3 +99,32 @@ def test_zero_step_case_returns_initial_state():
         rho, j = srv.fetch(cid)
     np.testing.assert_allclose(rho, 1.0, rtol=1e-6)
     assert np.abs(j).max() < 1e-6
+
+
+def test_failed_case_does_not_stop_the_others():
+    """A case whose setup fails (an infinite perturbation amplitude) is
+    reported FAILED with the reason, its memory returns to the arena, and the
+    cases around it complete."""
+    say("case: failed case in the mix")
+    with serving.Server(1e9, chunk_ms=0.5, max_chunk_steps=8) as srv:
+        good = [srv.submit(serving.make_case("cavity", (14, 14, 14), 12 + k, "f32", seed=k)) for k in range(2)]
+        bad = srv.submit(serving.make_case("cavity", (14, 14, 14), 10, "f32", perturbation=float("inf")))
+        done = {}
+        for _ in range(200):
+            srv.progress()
+            while True:
+                st = srv.next_done()
+                if st is None:
+                    break
+                done[st["id"]] = st["state"]
+            if len(done) == 3:
+                break
+        assert done[bad] == _capi.CASE_FAILED
+        assert all(done[g] == _capi.CASE_DONE for g in good)
+        with pytest.raises(LbmError, match="finite"):
+            srv.fetch(bad)
+        for g in good:
+            rho, _ = srv.fetch(g)
+            assert np.isfinite(rho).all()
+        st = srv.stats()
+        assert st["failed"] == 1 and st["completed"] == 2 and st["bytes_in_use"] == 0

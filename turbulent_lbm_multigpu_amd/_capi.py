@@ -46,6 +46,7 @@ CASE_CHANNEL = 1
 CASE_QUEUED = 0
 CASE_RUNNING = 1
 CASE_DONE = 2
+CASE_FAILED = 3
 
 EXPORTED_SYMBOLS = (
     "lbm_version", "lbm_last_error", "lbm_device_synchronize", "lbm_solver_config_default",
@@ -146,6 +147,7 @@ class ServerStats(C.Structure):
     _fields_ = [
         ("submitted", C.c_longlong),
         ("completed", C.c_longlong),
+        ("failed", C.c_longlong),
         ("chunks", C.c_longlong),
         ("steps_cells", C.c_longlong),
         ("bytes_in_use", C.c_size_t),

@@ -80,6 +80,7 @@ struct Case {
     long long cells = 0;
     double submit_s = 0, admit_s = 0, done_s = 0;
     std::vector<float> rho, j;
+    std::string error;  // why a FAILED case failed
 };
 
 struct Pool {
@@ -307,10 +308,23 @@ int admit(lbm_server *s) {
         s->stats.bytes_in_use += align256(c->bytes);
         s->stats.bytes_peak = std::max(s->stats.bytes_peak, s->stats.bytes_in_use);
         int rc = case_setup(s, c);
-        if (rc) return rc;
+        it = s->queue.erase(it);
+        if (rc) {
+            // the case fails on its own: its block returns to the arena and
+            // the server goes on with the others
+            if (c->solver) lbm_solver_destroy(c->solver);
+            c->solver = nullptr;
+            s->arena.release(c->offset, c->bytes);
+            s->stats.bytes_in_use -= align256(c->bytes);
+            c->state = LBM_CASE_FAILED;
+            c->error = s->err;
+            c->done_s = s->now();
+            s->done_order.push_back(c->id);
+            s->stats.failed++;
+            continue;
+        }
         c->admit_s = s->now();
         s->ready.push_back(c);
-        it = s->queue.erase(it);
     }
     return LBM_OK;
 }
@@ -552,6 +566,11 @@ extern "C" int lbm_server_fetch(lbm_server *s, long long id, float *rho, float *
     auto it = s->cases.find(id);
     if (it == s->cases.end()) return s->fail(LBM_ERR_OUT_OF_RANGE, "unknown case id");
     Case *c = it->second.get();
+    if (c->state == LBM_CASE_FAILED) {
+        const std::string why = "case " + std::to_string(id) + " failed: " + c->error;
+        s->cases.erase(it);
+        return s->fail(LBM_ERR_STATE, why);
+    }
     if (c->state != LBM_CASE_DONE) return s->fail(LBM_ERR_STATE, "case not completed");
     if (rho) std::copy(c->rho.begin(), c->rho.end(), rho);
     if (j) std::copy(c->j.begin(), c->j.end(), j);

@@ -205,12 +205,14 @@ class ServeResult:
     latencies: List[float] = field(default_factory=list)
     wall_s: float = 0.0
     completed: int = 0
+    failed: int = 0
+    errors: List[str] = field(default_factory=list)
     stats: dict = field(default_factory=dict)
     results: Dict[int, tuple] = field(default_factory=dict)
 
     def summary(self) -> dict:
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
-        return {"completed": self.completed, "wall_s": round(self.wall_s, 3),
+        return {"completed": self.completed, "failed": self.failed, "wall_s": round(self.wall_s, 3),
                 "cases_per_s": round(self.completed / self.wall_s, 3) if self.wall_s > 0 else 0.0,
                 "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
                 "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
@@ -252,12 +254,19 @@ def serve(server: Server, requests: Sequence[Request], *, storage: Optional[str]
             if st is None:
                 break
             cid = st["id"]
+            outstanding -= 1
+            if st["state"] == _capi.CASE_FAILED:
+                res.failed += 1
+                try:
+                    server.fetch(cid, keep=False)
+                except LbmError as e:
+                    res.errors.append(str(e))
+                continue
             res.latencies.append(st["done_s"] - arrivals[cid])
             last_done = max(last_done, st["done_s"])
             out = server.fetch(cid, keep=keep_results)
             if keep_results:
                 res.results[ids[cid]] = out
-            outstanding -= 1
             res.completed += 1
             if log:
                 log(f"  case {ids[cid]} done: {st['cells']} cells x {st['steps_done']} steps, "
