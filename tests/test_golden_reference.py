@@ -44,19 +44,21 @@ def test_oracle_matches_reference_kernels(path):
 
 @pytest.mark.skipif(not FILES, reason="no golden vectors committed")
 def test_golden_vectors_detect_a_planted_error():
-    """The comparison has teeth: the oracle with tau off by 1e-4 relative, or
-    with one extra step, misses the reference's fields by far more than 1e-5."""
+    """The comparison has teeth: the oracle with tau or the lid speed off by a
+    small relative amount, or read out in the wrong step parity (slot i vs
+    opp(i)), misses the reference's fields by far more than 1e-5."""
     path = [p for p in FILES if "general_lds" in p][0]
     z = np.load(path, allow_pickle=False)
     case = json.loads(bytes(z["params"]).decode())
     case["dims"], case["bc"], case["g"] = tuple(case["dims"]), tuple(case["bc"]), tuple(case["g"])
     fluid = z["flags"] == orc.FLAG_FLUID
-    for planted in ("tau", "steps"):
+    for planted in ("tau", "lid", "parity"):
         c = dict(case)
         if planted == "tau":
             c["tau"] = case["tau"] * (1 + 1e-4)
+        if planted == "lid":
+            c["lid"] = case["lid"] * (1 + 1e-3)
         o, _ = oracle_initial(c)
-        n = case["steps"] + (1 if planted == "steps" else 0)
-        o.run(n)
-        rho, j = moments(o.dd, n)
+        o.run(case["steps"])
+        rho, j = moments(o.dd, case["steps"] + (1 if planted == "parity" else 0))
         assert rel(j[:, fluid], z["j"][:, fluid].astype(np.float64)) > 10 * RTOL, planted
