@@ -24,8 +24,15 @@ def run(*args, cwd=None, timeout=120):
     return subprocess.run([EXE, *args], capture_output=True, text=True, timeout=timeout, cwd=cwd)
 
 
-def test_validation_mode_all_subdomains_match(tmp_path):
-    r = run("-c", os.path.join(DATA, "conf_validate_les.xml"), cwd=tmp_path)
+@pytest.mark.parametrize("fused_x", [0, 1])
+def test_validation_mode_all_subdomains_match(tmp_path, fused_x):
+    conf = os.path.join(DATA, "conf_validate_les.xml")
+    if fused_x:  # x faces exchanged inside the step kernel: interiors stay identical
+        text = open(conf).read().replace("<exchange-mode>1</exchange-mode>",
+                                         "<exchange-mode>1</exchange-mode><fused-x>1</fused-x>")
+        conf = str(tmp_path / "conf_fused.xml")
+        open(conf, "w").write(text)
+    r = run("-c", conf, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = re.findall(r"NUMBER OF FAILED CELLS/TOTAL NUMBER OF CELLS: (\d+)/(\d+)", r.stdout)
     assert len(lines) == 8

@@ -6,7 +6,7 @@
 //   <lattice> <tau/> <lid-velocity/> <body-force><x/><y/><z/></body-force>
 //             <smagorinsky-cs/> <storage>f32|f64|f16|bf16</storage>
 //             <periodic><x/><y/><z/></periodic> <forcing>reference|body</forcing>
-//             <kernel-variant/> <exchange-mode/> <perturbation><amplitude/><seed/></perturbation>
+//             <kernel-variant/> <exchange-mode/> <fused-x/> <perturbation><amplitude/><seed/></perturbation>
 //   </lattice>
 #pragma once
 
@@ -143,6 +143,7 @@ public:
     double perturbation_amplitude = 0.0;
     unsigned long long perturbation_seed = 0;
     int exchange_mode = 0;  // 0: all 19 populations after alpha (reference), 1: the 5 an inner cell reads
+    int fused_x = 0;        // 1: x faces exchanged inside the step kernel
 
     CConfiguration() {}
     explicit CConfiguration(const std::string &file_name) { loadFile(file_name); }
@@ -225,6 +226,7 @@ private:
             if (auto *c = lat->child("forcing")) lattice.forcing = c->text == "body" ? LBM_FORCING_BODY : LBM_FORCING_REFERENCE;
             if (auto *c = lat->child("kernel-variant")) lattice.kernel_variant = atoi(c->text.c_str());
             if (auto *c = lat->child("exchange-mode")) exchange_mode = atoi(c->text.c_str());
+            if (auto *c = lat->child("fused-x")) fused_x = atoi(c->text.c_str());
             if (auto *c = lat->child("storage")) {
                 const std::string &v = c->text;
                 lattice.storage = v == "f64" ? LBM_STORAGE_F64 : v == "f16" ? LBM_STORAGE_F16_SHIFTED

@@ -148,6 +148,7 @@ public:
         lbm_group_set_exchange_mode(_group, 0);
         lbm_group_sync(_group, 0);
         lbm_group_set_exchange_mode(_group, cfg->exchange_mode);
+        lbm_group_set_fused_x(_group, cfg->fused_x);
         for (auto *ctl : _controllers) ctl->attachGroup(_group);
     }
 

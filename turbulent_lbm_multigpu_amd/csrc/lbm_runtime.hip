@@ -496,7 +496,9 @@ hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int 
         auto pick = [&](auto b) {
             switch (v) {
                 case 0: go(b, std::integral_constant<int, 0>()); break;
+                case 1: go(b, std::integral_constant<int, 1>()); break;
                 case 2: go(b, std::integral_constant<int, 2>()); break;
+                case 3: go(b, std::integral_constant<int, 3>()); break;
                 case 8: go(b, std::integral_constant<int, 8>()); break;
                 case 16: go(b, std::integral_constant<int, 16>()); break;
                 default: go(b, std::integral_constant<int, 0>()); break;
@@ -648,10 +650,10 @@ static int validate_config(const lbm_solver_config &c) {
     if (c.smagorinsky_cs < 0.0 || (c.smagorinsky_cs > 0.0 && !(c.tau > 0.5)))
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "LES needs smagorinsky_cs >= 0 and tau > 0.5");
     const int v = c.kernel_variant & ~4;
-    if (c.kernel_variant < 0 || (v != 0 && v != 2 && v != 8 && v != 16) || c.lds_per_block < 0 ||
+    if (c.kernel_variant < 0 || (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 16) || c.lds_per_block < 0 ||
         c.lds_per_block > 65536)
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
-                      "kernel_variant must be one of 0, 2, 8, 16 (optionally | 4); lds_per_block in [0, 65536]");
+                      "kernel_variant must be one of 0, 1, 2, 3, 8, 16 (optionally | 4); lds_per_block in [0, 65536]");
     return LBM_OK;
 }
 
@@ -1395,7 +1397,7 @@ extern "C" int lbm_group_create_empty(int device, int storage, int work_group_si
     *out = nullptr;
     const int v = kernel_variant & ~4;
     if (storage < 0 || storage > 3 || work_group_size < 64 || work_group_size > 256 || work_group_size % 64 != 0 ||
-        kernel_variant < 0 || (v != 0 && v != 2 && v != 8 && v != 16))
+        kernel_variant < 0 || (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 16))
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad storage or launch shape");
     lbm_group *g = new lbm_group();
     g->storage = storage;
@@ -1754,7 +1756,7 @@ extern "C" int lbm_group_set_launch(lbm_group *g, int work_group_size, int kerne
     if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
     const int v = kernel_variant & ~4;
     if (work_group_size < 64 || work_group_size > 256 || work_group_size % 64 != 0 || kernel_variant < 0 ||
-        (v != 0 && v != 2 && v != 8 && v != 16))
+        (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 16))
         return gfail(g, LBM_ERR_INVALID_ARGUMENT, "unsupported launch shape");
     GCHK(hipStreamSynchronize(g->stream));
     g->wg = work_group_size;

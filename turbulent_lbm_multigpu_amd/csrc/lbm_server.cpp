@@ -378,7 +378,7 @@ extern "C" int lbm_server_create(const lbm_server_config *cfg, lbm_server **out)
     const int v = cfg->kernel_variant & ~4;
     if (cfg->memory_budget < ((size_t)1 << 20) || cfg->max_active < 0 || !(cfg->chunk_ms > 0.0) ||
         cfg->max_chunk_steps < 2 || cfg->work_group_size < 64 || cfg->work_group_size > 256 ||
-        cfg->work_group_size % 64 != 0 || cfg->kernel_variant < 0 || (v != 0 && v != 2 && v != 8 && v != 16))
+        cfg->work_group_size % 64 != 0 || cfg->kernel_variant < 0 || (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 16))
         return LBM_ERR_INVALID_ARGUMENT;
     std::unique_ptr<lbm_server> s(new lbm_server());
     s->cfg = *cfg;
