@@ -194,7 +194,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build runs every subdomain on a single device "
+        print("bench.py measures one GPU: every subdomain runs on a single device "
               "(no multi-GPU transport); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
 

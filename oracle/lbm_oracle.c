@@ -2,8 +2,8 @@
  * lbm_oracle.c -- CPU oracle for the D3Q19 AA-pattern hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg load this library (as the checker, or as the
- * timed CPU baseline).  The product library never links or calls it.
+ * bench.py's cpu_baseline leg load this library (as what the tests compare
+ * against, or as the timed CPU baseline).  The product library never links or calls it.
  *
  * It restates, as plain scalar C:
  *   init_kernel       src/cl_programs/lbm_init.cl:32-237

@@ -4,8 +4,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a scalar CPU restatement of the
  * reference's D3Q19 AA-pattern kernels.  Only tests/, __graft_entry__.smoke()
- * and bench.py's cpu_baseline leg may load it, and only as the checker or the
- * timed CPU baseline.  The product path never calls it.
+ * and bench.py's cpu_baseline leg may load it, and only as the reference the
+ * tests compare against or the timed CPU baseline.  The product path never calls it.
  *
  * Every function names the reference file:line it follows.  Arithmetic is
  * written as separate IEEE operations in the reference's evaluation order and

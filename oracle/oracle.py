@@ -1,7 +1,8 @@
 """ctypes front end of the CPU oracle (oracle/liblbm_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.
+bench.py's cpu_baseline leg, as what the tests compare against or as the
+timed CPU baseline.
 
 ``OracleDomain`` holds one (sub)domain in host memory (numpy) and steps it
 with the C restatement of the reference kernels; ``OracleMulti`` runs a

@@ -124,7 +124,7 @@ int main(int argc, char **argv) {
         cfg->debug_mode = debug;
         if (debug) cfg->printMe();
         if (unit_test) {
-            std::cout << "unit tests of this build run with pytest (tests/); suite '" << test_suite << "' ignored"
+            std::cout << "unit tests run with pytest (tests/); suite '" << test_suite << "' ignored"
                       << std::endl;
             return 0;
         }
