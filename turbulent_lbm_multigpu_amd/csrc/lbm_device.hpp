@@ -528,12 +528,12 @@ __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int ce
 #pragma unroll
         for (int i = 0; i < 18; ++i) {
             const unsigned idx = wrap ? (unsigned)wrap_idx((int)g - d[i], cells) : g - (unsigned)d[i];
-            f[i] = kld<SP, NTL>(pl[opp(i)], cell_base(idx) * SP::kBytes, opp(i));
+            f[i] = kld<SP>(pl[opp(i)], cell_base(idx) * SP::kBytes, opp(i));
         }
-        f[18] = kld<SP, NTL>(pl[18], own, 18);
+        f[18] = kld<SP>(pl[18], own, 18);
     } else {
         // the 19 loads do not wait for the flag; a GHOST cell (lbm_alpha.cl:30-32)
-        // matches no branch of cell_update and stores nothing
+        // matches no branch of cell_update and stores its slots back unchanged
 #pragma unroll
         for (int i = 0; i < 19; ++i) f[i] = kld<SP, NTL>(pl[i], own, i);
     }
@@ -559,9 +559,18 @@ __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int ce
                            i, f[i]);
             }
             kst<SP, NTS>(pl[18], own, 18, f[18]);
-        } else {
+        }
+    }
+    if (!BETA) {
+        // Every slot of the cell is stored: slot k takes f[opp(k)] where the
+        // cell collided, else its loaded value f[k] (the reference stores
+        // nothing there, lbm_alpha.cl:305-343; rewriting the same value keeps
+        // memory identical and every cache line fully written).  A ghost cell
+        // that a fused neighbour is feeding in this launch stays untouched.
+        const bool fed = !write_dd && x >= 0 && ((x == 0 && D->xrecv[0]) || (x == X - 1 && D->xrecv[1]));
+        if (!fed) {
 #pragma unroll
-            for (int i = 0; i < 19; ++i) kst<SP, NTS>(pl[opp(i)], own, opp(i), f[i]);
+            for (int k = 0; k < 19; ++k) kst<SP, NTS>(pl[k], own, k, write_dd ? f[opp(k)] : f[k]);
         }
     }
     if (!BETA && x >= 0) {
@@ -631,39 +640,34 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
         else vld<SP, N>(pl[src], own, src, F[i]);
     }
     T vx[N], vy[N], vz[N], dn[N];
-    bool all_write = true, any_ghost = false;
+    bool wr[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         T f[19];
 #pragma unroll
         for (int i = 0; i < 19; ++i) f[i] = F[i][j];
-        const bool w = cell_update<T, BETA, SP::kShifted>(f, flag[j], D, vx[j], vy[j], vz[j], dn[j]);
-        all_write = all_write && w;
-        any_ghost = any_ghost || flag[j] == kFlagGhost;
+        wr[j] = cell_update<T, BETA, SP::kShifted>(f, flag[j], D, vx[j], vy[j], vz[j], dn[j]);
 #pragma unroll
-        for (int i = 0; i < 19; ++i) F[i][j] = w ? f[i] : F[i][j];
-        if (!BETA && !w) flag[j] = -flag[j] - 1;  // remember: this cell stores nothing
+        for (int i = 0; i < 19; ++i) F[i][j] = wr[j] ? f[i] : F[i][j];
     }
     if (BETA) {
 #pragma unroll
         for (int i = 0; i < 18; ++i) vst_any<SP, N>(pl[i], g0 + (unsigned)d[i], i, F[i]);
         vst<SP, N>(pl[18], own, 18, F[18]);
-    } else if (all_write) {
-#pragma unroll
-        for (int i = 0; i < 19; ++i) vst<SP, N>(pl[opp(i)], own, opp(i), F[i]);
     } else {
+        // every slot stored: f[opp(k)] where the cell collided, else the
+        // loaded value (see cell_scalar)
 #pragma unroll
-        for (int j = 0; j < N; ++j)
-            if (flag[j] >= 0)
+        for (int k = 0; k < 19; ++k) {
+            T v[N];
 #pragma unroll
-                for (int i = 0; i < 19; ++i) kst<SP>(pl[opp(i)], own + j * es, opp(i), F[i][j]);
+            for (int j = 0; j < N; ++j) v[j] = wr[j] ? F[opp(k)][j] : F[k][j];
+            vst<SP, N>(pl[k], own, k, v);
+        }
     }
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const int fl = flag[j] < 0 ? -flag[j] - 1 : flag[j];
-        if (fl != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx[j], vy[j], vz[j], dn[j]);
-    }
-    (void)any_ghost;
+    for (int j = 0; j < N; ++j)
+        if (flag[j] != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx[j], vy[j], vz[j], dn[j]);
 }
 
 template <int ST, bool BETA, int VAR = 0>
