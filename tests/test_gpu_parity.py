@@ -193,9 +193,10 @@ def test_mass_energy_fp64_reduction():
     s.close()
 
 
-def build_gpu_group(domain_size, nums, *, lid, amp, seed, storage=_capi.STORAGE_F32, mode=0, graph=True, **kw):
+def build_gpu_group(domain_size, nums, *, lid, amp, seed, storage=_capi.STORAGE_F32, mode=0, graph=True,
+                    periodic=(False, False, False), **kw):
     mono = dec.monolithic_size(domain_size, nums)
-    subs = dec.decompose(domain_size, nums)
+    subs = dec.decompose(domain_size, nums, periodic=periodic)
     solvers = []
     for sd in subs:
         bc = [sd.bc[0][0], sd.bc[0][1], sd.bc[1][0], sd.bc[1][1], sd.bc[2][0], sd.bc[2][1]]
@@ -298,27 +299,83 @@ def fluid_slots(s: LbmSolver):
     return gpu_dd(s).reshape(19, -1)[:, fl == orc.FLAG_FLUID]
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("domain_size,nums", [((28, 28, 28), (2, 2, 2)), ((36, 16, 24), (3, 1, 2))])
-def test_fused_x_faces_match_copy_exchange(mode, domain_size, nums):
-    """x faces exchanged inside the step kernel: every FLUID cell's slots are
-    bit-identical to the copy-based exchange, after odd and even step counts."""
-    say(f"case: fused x faces {domain_size} {nums} mode {mode}")
-    kw = dict(inv_tau=1 / 0.52, tau=0.52, smagorinsky_cs=0.17)
-    ref, subs, _ = build_gpu_group(domain_size, nums, lid=0.08, amp=1e-3, seed=11, mode=mode, **kw)
-    fus, _, _ = build_gpu_group(domain_size, nums, lid=0.08, amp=1e-3, seed=11, mode=mode, **kw)
+def fused_vs_copy(domain_size, nums, mode, *, wg=256, periodic=(False, False, False), stops=(), step_runs=(5, 8)):
+    """Two identical groups, one with x faces inside the step kernel: every
+    slot of every subdomain is bit-identical after each run of steps."""
+    kw = dict(inv_tau=1 / 0.52, tau=0.52, smagorinsky_cs=0.17, work_group_size=wg)
+    ref, _, _ = build_gpu_group(domain_size, nums, lid=0.08, amp=1e-3, seed=11, mode=mode, periodic=periodic, **kw)
+    fus, _, _ = build_gpu_group(domain_size, nums, lid=0.08, amp=1e-3, seed=11, mode=mode, periodic=periodic, **kw)
     fus.set_fused_x(True)
-    for steps in (5, 8):
+    for member, at in stops:
+        ref.set_member_stop(member, at)
+        fus.set_member_stop(member, at)
+    for steps in step_runs:
         ref.step(steps)
         fus.step(steps)
         ref.wait()
         fus.wait()
         for a, b in zip(ref.solvers, fus.solvers):
-            np.testing.assert_array_equal(fluid_slots(a), fluid_slots(b))
+            np.testing.assert_array_equal(gpu_dd(a), gpu_dd(b))
     for g in (ref, fus):
         g.close()
         for s in g.solvers:
             s.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("domain_size,nums", [((28, 28, 28), (2, 2, 2)), ((36, 16, 24), (3, 1, 2)),
+                                              ((56, 12, 10), (4, 1, 1))])
+def test_fused_x_faces_match_copy_exchange(mode, domain_size, nums):
+    """x faces exchanged inside the step kernel (chains of 2, 3 and 4
+    subdomains along x): all slots bit-identical to the copy-based exchange,
+    after odd and even step counts."""
+    say(f"case: fused x faces {domain_size} {nums} mode {mode}")
+    fused_vs_copy(domain_size, nums, mode)
+
+
+@pytest.mark.parametrize("wg", [64, 128, 256])
+@pytest.mark.parametrize("domain_size", [(56, 12, 10), (200, 12, 10), (260, 12, 10)])
+def test_fused_x_chain_fits_work_group_or_copies(wg, domain_size):
+    """A chain tile holds one row of every member; chains wider than the
+    work-group keep the copy exchange.  Results never depend on which."""
+    nums = (4, 1, 1) if domain_size[0] == 56 else (2, 1, 1)
+    say(f"case: fused x chain {domain_size} {nums} wg {wg}")
+    fused_vs_copy(domain_size, nums, 1, wg=wg, step_runs=(3, 4))
+
+
+def test_fused_x_periodic_ring_of_two():
+    """x periodic across two subdomains: a ring chain (each member feeds
+    both faces of the other)."""
+    say("case: fused x periodic ring (2,1,1)")
+    fused_vs_copy((36, 14, 12), (2, 1, 1), 1, periodic=(True, False, False), step_runs=(5, 6))
+
+
+def test_fused_x_with_frozen_member():
+    """A member frozen at its own step count inside a chain: the frozen
+    sender's slots as they stand feed its partner, as the copies do."""
+    say("case: fused x chain with a frozen member")
+    fused_vs_copy((42, 12, 10), (3, 1, 1), 1, stops=((1, 3), (2, 6)), step_runs=(5, 6))
+
+
+def test_group_2x2x2_fused_matches_oracle_decomposition():
+    """The fused x faces against the oracle's copy-based decomposition (all
+    slots, exchange of all 19 populations)."""
+    say("case: fused group 2x2x2 of 14^3 vs oracle")
+    from tests.test_oracle import build_oracle_decomposition
+    domain_size, nums = (28, 28, 28), (2, 2, 2)
+    kw = dict(inv_tau=1 / 0.5005, tau=0.5005, smagorinsky_cs=0.17)
+    multi, _ = build_oracle_decomposition(domain_size, nums, lid=0.1, amp=1e-4, seed=20260101, **kw)
+    grp, _, _ = build_gpu_group(domain_size, nums, lid=0.1, amp=1e-4, seed=20260101, **kw)
+    grp.set_fused_x(True)
+    for steps in (3, 6):
+        multi.run(steps)
+        grp.step(steps)
+        grp.wait()
+        for sub, s in zip(multi.subs, grp.solvers):
+            np.testing.assert_array_equal(gpu_dd(s), sub["domain"].dd)
+    grp.close()
+    for s in grp.solvers:
+        s.close()
 
 
 def test_fused_x_periodic_channel():

@@ -242,12 +242,24 @@ struct DomDev {
     int store_density;
     double inv_tau, tau0, les_coeff, gx, gy, gz, lid;
     long long stop_at;  // launch half-step index from which this domain stays frozen
-    // Fused x-face exchange (same-sized neighbours across x): the step kernel
-    // writes the face values straight into the neighbour instead of an
-    // exchange launch copying them afterwards.
-    void *xnb[2];          // receiver across this domain's x- / x+ face, or null
-    unsigned xmask_a[2];   // slots that receiver's ghost layer takes after alpha
-    int xrecv[2];          // this domain's x- / x+ ghost layer is fed that way
+    unsigned xmask_a[2];  // slots this domain's x- / x+ ghost layer takes after alpha in a chain tile
+};
+
+// ---- fused x chains ----------------------------------------------------------
+// Same-sized subdomains joined across x faces form a chain (a line, or a ring
+// when x is periodic).  A chain tile holds the same `rows` (y, z) rows of every
+// member, so both columns next to each shared face are in one work-group: the
+// face values move through LDS and each population slot is stored only by the
+// work-item that owns it in the AA pattern (whole cache lines, no stores into
+// another subdomain), with the result of the copy-based exchange.
+constexpr int kMaxChain = 8;    // members of one chain
+constexpr int kChainRows = 16;  // members x rows of one chain tile (LDS capacity)
+struct XChain {
+    int nmem, rows, nx, tile_begin;
+    int lanes;             // work-items per member: rows * nx rounded up to whole wavefronts
+    int mem[kMaxChain];    // descriptor index of each position
+    int left[kMaxChain];   // position whose x = X-2 column feeds my x = 0 ghost layer, or -1
+    int right[kMaxChain];  // position whose x = 1 column feeds my x = X-1 ghost layer, or -1
 };
 
 // ---- collision arithmetic (same operation order as the oracle) -------------
@@ -517,13 +529,10 @@ __device__ __forceinline__ void store_macros(const DomDev *D, unsigned g, int ce
 // One cell, element accesses.  With WRAP the neighbour index goes through the
 // linear wrap W (src/cl_programs/wrap.h:112-127); without it the block is far
 // enough from both ends of the domain for W to be the identity.
-template <class SP, bool BETA, bool NTS, bool NTL = false>
-__device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
-                                            const DomDev *D, int x = -1) {
-    using T = typename SP::T;
-    const int flag = ((const guchar *)D->flags)[g];
+template <class SP, bool BETA, bool NTL>
+__device__ __forceinline__ void cell_load(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
+                                          typename SP::T *f) {
     const unsigned own = cell_base(g) * SP::kBytes;
-    T f[19];
     if (BETA) {
 #pragma unroll
         for (int i = 0; i < 18; ++i) {
@@ -532,63 +541,56 @@ __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int ce
         }
         f[18] = kld<SP>(pl[18], own, 18);
     } else {
-        // the 19 loads do not wait for the flag; a GHOST cell (lbm_alpha.cl:30-32)
-        // matches no branch of cell_update and stores its slots back unchanged
+        // the 19 loads do not wait for the flag
 #pragma unroll
         for (int i = 0; i < 19; ++i) f[i] = kld<SP, NTL>(pl[i], own, i);
     }
+}
+
+// Beta scatters f[i] to slot i of W(g + d_i) (lbm_beta.cl:708-784).  Alpha
+// stores every slot of its cell: f[] already holds slot k's value at k.
+template <class SP, bool BETA, bool NTS>
+__device__ __forceinline__ void cell_store(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
+                                           const typename SP::T *f) {
+    const unsigned own = cell_base(g) * SP::kBytes;
+    if (BETA) {
+#pragma unroll
+        for (int i = 0; i < 18; ++i) {
+            const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
+            kst<SP, NTS>(pl[i], cell_base(idx) * SP::kBytes, i, f[i]);
+        }
+        kst<SP, NTS>(pl[18], own, 18, f[18]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 19; ++k) kst<SP, NTS>(pl[k], own, k, f[k]);
+    }
+}
+
+// Alpha's slot values: slot k takes f[opp(k)] where the cell collided, else
+// its loaded value f[k] (the reference stores nothing there,
+// lbm_alpha.cl:305-343; rewriting the same value keeps memory identical and
+// every cache line fully written).
+template <typename T>
+__device__ __forceinline__ void alpha_slots(T *f, bool write_dd) {
+#pragma unroll
+    for (int k = 0; k < 18; k += 2) {
+        const T a = f[k], b = f[k + 1];
+        f[k] = write_dd ? b : a;
+        f[k + 1] = write_dd ? a : b;
+    }
+}
+
+template <class SP, bool BETA, bool NTS, bool NTL = false>
+__device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
+                                            const DomDev *D) {
+    using T = typename SP::T;
+    const int flag = ((const guchar *)D->flags)[g];
+    T f[19];
+    cell_load<SP, BETA, NTL>(pl, g, cells, d, wrap, f);
     T vx, vy, vz, dens;
     const bool write_dd = cell_update<T, BETA, SP::kShifted>(f, flag, D, vx, vy, vz, dens);
-    const int X = D->nx;
-    if (write_dd) {
-        if (BETA) {
-            // fused x faces (x >= 0): a ghost cell of a fed face scatters no
-            // population with an x component (those slots are the sender's,
-            // and what it gathered for them may already be the sender's new
-            // value); a cell next to a feeding face also writes the receiver
-            const bool skip_x = x >= 0 && ((x == 0 && D->xrecv[0]) || (x == X - 1 && D->xrecv[1]));
-#pragma unroll
-            for (int i = 0; i < 18; ++i) {
-                const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
-                if (!(ex(i) != 0 && skip_x)) kst<SP, NTS>(pl[i], cell_base(idx) * SP::kBytes, i, f[i]);
-                if (ex(i) == 1 && x == X - 2 && D->xnb[1])
-                    kst<SP>((gchar *)D->xnb[1] + (size_t)i * kBlk * SP::kBytes, cell_base(idx - X + 2) * SP::kBytes,
-                           i, f[i]);
-                if (ex(i) == -1 && x == 1 && D->xnb[0])
-                    kst<SP>((gchar *)D->xnb[0] + (size_t)i * kBlk * SP::kBytes, cell_base(idx + X - 2) * SP::kBytes,
-                           i, f[i]);
-            }
-            kst<SP, NTS>(pl[18], own, 18, f[18]);
-        }
-    }
-    if (!BETA) {
-        // Every slot of the cell is stored: slot k takes f[opp(k)] where the
-        // cell collided, else its loaded value f[k] (the reference stores
-        // nothing there, lbm_alpha.cl:305-343; rewriting the same value keeps
-        // memory identical and every cache line fully written).  A ghost cell
-        // that a fused neighbour is feeding in this launch stays untouched.
-        const bool fed = !write_dd && x >= 0 && ((x == 0 && D->xrecv[0]) || (x == X - 1 && D->xrecv[1]));
-        if (!fed) {
-#pragma unroll
-            for (int k = 0; k < 19; ++k) kst<SP, NTS>(pl[k], own, k, write_dd ? f[opp(k)] : f[k]);
-        }
-    }
-    if (!BETA && x >= 0) {
-        // the receiver's ghost cell across a fused x face takes this cell's
-        // slots as they stand after alpha: slot k holds f[opp(k)] if the cell
-        // stored, else its loaded value f[k]
-        const int side = x == 1 ? 0 : (x == X - 2 ? 1 : -1);
-        if (side >= 0 && D->xnb[side]) {
-            gchar *rb = (gchar *)D->xnb[side];
-            const unsigned r = side ? g - X + 2 : g + X - 2;
-            const unsigned m = D->xmask_a[side];
-#pragma unroll
-            for (int k = 0; k < 19; ++k)
-                if (m & (1u << k))
-                    kst<SP>(rb + (size_t)k * kBlk * SP::kBytes, cell_base(r) * SP::kBytes, k,
-                           write_dd ? f[opp(k)] : f[k]);
-        }
-    }
+    if (!BETA) alpha_slots(f, write_dd);
+    cell_store<SP, BETA, NTS>(pl, g, cells, d, wrap, f);  // beta stores every cell
     if (flag == kFlagGhost) return;  // lbm_alpha.cl:30-32, lbm_beta.cl:787-788
     store_macros<T>(D, g, cells, vx, vy, vz, dens);
 }
@@ -670,9 +672,106 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
         if (flag[j] != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx[j], vy[j], vz[j], dn[j]);
 }
 
+// One chain tile: position m of the chain, row r of the tile and column x
+// per work-item (whole rows, so the x = X-2 / x = 1 columns feeding a ghost
+// layer are in this work-group).  Senders put their face values in LDS; after
+// the barrier a ghost cell of a joined face takes them in place of its own:
+//   alpha: the slots its receiver mask names (the sender's slot values after
+//          alpha, src/CController.hpp:265-320);
+//   beta:  the populations it scatters into the inner layer, n.e > 0 (the
+//          sender's values scattered into its own ghost layer,
+//          src/CController.hpp:322-383).
+// A frozen member stores nothing; as a sender it feeds its slots as they
+// stand, which is what the copy exchange reads from it.  Each member has its
+// own whole wavefronts, so the member and every address base derived from it
+// stay uniform per wavefront.
+template <class SP, bool BETA, bool NTS, bool NTL>
+__device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, const XChain *__restrict__ C, int t,
+                                           long long h) {
+    using T = typename SP::T;
+    __shared__ T xv[kChainRows * 2 * 19];
+    const int X = C->nx, R = C->rows, span = R * X;
+    const int q = (int)threadIdx.x;
+    const int m = __builtin_amdgcn_readfirstlane(q / C->lanes);
+    const int r = (q - m * C->lanes) / X;
+    const int x = q - m * C->lanes - r * X;
+    const bool lane = m < C->nmem && r < R;
+    const DomDev *D = doms + C->mem[m < C->nmem ? m : 0];
+    const int cells = (int)D->cells;
+    const int gfirst = (t - C->tile_begin) * span;  // first cell of the tile's rows (equal dims along the chain)
+    const int g = gfirst + r * X + x;
+    const bool in = lane && g < cells;
+    const bool live = in && h < D->stop_at;
+    T f[19];
+    int flag = kFlagGhost;
+    T vx = T(0), vy = T(0), vz = T(0), dens = T(0);
+    gchar *pl[19];
+    {
+        gchar *base = (gchar *)D->dd;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) pl[i] = base + (size_t)i * kBlk * SP::kBytes;
+    }
+    int d[19];
+    bool wrap = false;
+    if (BETA) {
+        const int XY = X * D->ny;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) d[i] = ex(i) + X * ey(i) + XY * ez(i);
+        const int dmax = 1 + X + XY;
+        wrap = gfirst < dmax || gfirst + span + dmax > cells;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) d[i] = 0;
+    }
+    // senders: x = X-2 feeds the x = 0 ghost layer of the right partner,
+    // x = 1 the x = X-1 ghost layer of the left one
+    const int to = !in ? -1 : x == X - 2 ? C->right[m] : (x == 1 ? C->left[m] : -1);
+    const int sside = x == X - 2 ? 0 : 1;
+    T *so = xv + (((to < 0 ? 0 : to) * R + r) * 2 + sside) * 19;
+    if (live) {
+        flag = ((const guchar *)D->flags)[g];
+        cell_load<SP, BETA, NTL>(pl, (unsigned)g, cells, d, wrap, f);
+        const bool write_dd = cell_update<T, BETA, SP::kShifted>(f, flag, D, vx, vy, vz, dens);
+        if (!BETA) alpha_slots(f, write_dd);
+        if (to >= 0) {
+#pragma unroll
+            for (int k = 0; k < 19; ++k)
+                if (!BETA || ex(k) == (sside ? -1 : 1)) so[k] = f[k];
+        }
+    } else if (to >= 0) {
+        // frozen sender: alpha's slot values are its own slots, beta's are the
+        // slots its scatter would write
+#pragma unroll
+        for (int k = 0; k < 19; ++k) {
+            if (!BETA) {
+                so[k] = kld<SP>(pl[k], cell_base((unsigned)g) * SP::kBytes, k);
+            } else if (ex(k) == (sside ? -1 : 1)) {
+                const unsigned idx = wrap ? (unsigned)wrap_idx(g + d[k], cells) : (unsigned)(g + d[k]);
+                so[k] = kld<SP>(pl[k], cell_base(idx) * SP::kBytes, k);
+            }
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    const int from = x == 0 ? C->left[m] : (x == X - 1 ? C->right[m] : -1);
+    if (from >= 0) {
+        const int side = x == 0 ? 0 : 1;
+        const T *o = xv + ((m * R + r) * 2 + side) * 19;
+        const unsigned mask = D->xmask_a[side];
+#pragma unroll
+        for (int k = 0; k < 19; ++k) {
+            const bool take = BETA ? ex(k) == (side ? -1 : 1) : ((mask >> k) & 1u) != 0;
+            if (take) f[k] = o[k];
+        }
+    }
+    cell_store<SP, BETA, NTS>(pl, (unsigned)g, cells, d, wrap, f);
+    if (flag == kFlagGhost) return;
+    store_macros<T>(D, (unsigned)g, cells, vx, vy, vz, dens);
+}
+
 template <int ST, bool BETA, int VAR = 0>
 __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ doms, const int *__restrict__ tile_dom,
-                                                    int xcd_map, long long *ctr) {
+                                                    const XChain *__restrict__ chains, int xcd_map, long long *ctr) {
     using SP = Storage<ST>;
     constexpr bool NTS = (VAR & kVarNtStore) != 0;
     constexpr bool NTL = (VAR & kVarNtLoad) != 0;
@@ -688,7 +787,12 @@ __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ do
     }
     const int nb = (int)gridDim.x;
     const int t = xcd_map ? xcd_tile((int)blockIdx.x, nb) : (int)blockIdx.x;
-    const DomDev *D = doms + tile_dom[t];
+    const int td = tile_dom[t];
+    if (td < 0) {  // a tile of fused x chain -td-1 (uniform per work-group: the barrier is reached by all)
+        chain_tile<SP, BETA, NTS, NTL>(doms, chains + (-td - 1), t, h);
+        return;
+    }
+    const DomDev *D = doms + td;
     if (h >= D->stop_at) return;  // a finished member of a continuous batch keeps its state
     const int cells = (int)D->cells;
     const int block_cells = (int)blockDim.x * N;
@@ -713,25 +817,13 @@ __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ do
 #pragma unroll
         for (int i = 0; i < 19; ++i) d[i] = 0;
     }
-    // fused x faces: lanes holding a cell of the two outer columns on either
-    // side take the element path, which knows the x coordinate
-    const bool xf = D->xnb[0] || D->xnb[1] || D->xrecv[0] || D->xrecv[1];
-    int x0 = -1;
-    if (xf) x0 = g0 % D->nx;
-    const bool x_edge = xf && (x0 <= 1 || x0 + N - 1 >= D->nx - 2);
-    if (N > 1 && !wrap && g0 + N <= cells && !x_edge) {
+    if (N > 1 && !wrap && g0 + N <= cells) {
         cells_vector<SP, BETA, N>(pl, (unsigned)g0, cells, d, D);
         return;
     }
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        int xj = -1;
-        if (xf) {
-            xj = x0 + j;
-            if (xj >= D->nx) xj -= D->nx;
-        }
-        if (g0 + j < cells) cell_scalar<SP, BETA, NTS, NTL>(pl, (unsigned)(g0 + j), cells, d, wrap, D, xj);
-    }
+    for (int j = 0; j < N; ++j)
+        if (g0 + j < cells) cell_scalar<SP, BETA, NTS, NTL>(pl, (unsigned)(g0 + j), cells, d, wrap, D);
 }
 
 }  // namespace lbm

@@ -399,11 +399,13 @@ struct lbm_group {
     int nops[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     int nblocks[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     struct XFuse {
-        void *xnb[2];
-        unsigned xmask_a[2];
-        int xrecv[2];
+        int nb[2];          // member whose column feeds my x- / x+ ghost layer in the step kernel, or -1
+        unsigned mask[2];   // slots that ghost layer takes after alpha
     };
-    std::vector<XFuse> xfuse;  // per member, from the plan
+    std::vector<XFuse> xfuse;                 // per member, from the plan
+    std::vector<std::vector<int>> chains;     // joined members in x order, from the plan
+    XChain *d_chains = nullptr;
+    size_t chains_cap = 0;
     long long step_counter = 0;
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
@@ -483,7 +485,8 @@ int upload_desc(lbm_solver *s) {
 // template arguments; bit 2 of the variant turns the XCD-contiguous tile map
 // off; lds_per_block reserves LDS to cap the number of resident work-groups.
 hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
-                              const int *tile_table, hipStream_t stream, long long *ctr = nullptr) {
+                              const int *tile_table, hipStream_t stream, long long *ctr = nullptr,
+                              const XChain *chains = nullptr) {
     const int xcd = (variant & 4) ? 0 : 1;
     const int v = variant & ~4;
     dispatch_storage(storage, [&](auto st) -> int {
@@ -491,7 +494,7 @@ hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int 
         auto go = [&](auto b, auto vv) {
             constexpr bool B = decltype(b)::value;
             constexpr int V = decltype(vv)::value;
-            step_kernel<ST, B, V><<<tiles, wg, lds, stream>>>(descs, tile_table, xcd, ctr);
+            step_kernel<ST, B, V><<<tiles, wg, lds, stream>>>(descs, tile_table, chains, xcd, ctr);
         };
         auto pick = [&](auto b) {
             switch (v) {
@@ -1296,31 +1299,74 @@ static hipError_t grow_table(T *&ptr, size_t &cap, size_t n) {
 // members and launch shape (work-group size and cells per lane); also
 // re-publishes the host half-step count to the device counter.
 static int group_upload_descs(lbm_group *g) {
-    std::vector<DomDev> descs;
+    const size_t nm = g->members.size();
+    std::vector<DomDev> descs(nm);
     std::vector<int> tiles;
+    std::vector<XChain> xch;
     const long long per_tile = (long long)g->wg * cells_per_lane(g->variant);
-    for (size_t i = 0; i < g->members.size(); ++i) {
+    // members of a chain (plan): chain index and position
+    std::vector<int> chain_of(nm, -1), pos(nm, -1);
+    for (size_t c = 0; c < g->chains.size(); ++c)
+        for (size_t k = 0; k < g->chains[c].size(); ++k) {
+            chain_of[g->chains[c][k]] = (int)c;
+            pos[g->chains[c][k]] = (int)k;
+        }
+    for (size_t i = 0; i < nm; ++i) {
         lbm_solver *m = g->members[i];
         fill_desc(m);
-        DomDev d = m->host_desc;
-        if (i < g->xfuse.size()) {
-            for (int side = 0; side < 2; ++side) {
-                d.xnb[side] = g->xfuse[i].xnb[side];
-                d.xmask_a[side] = g->xfuse[i].xmask_a[side];
-                d.xrecv[side] = g->xfuse[i].xrecv[side];
-            }
+        descs[i] = m->host_desc;
+        if (i < g->xfuse.size())
+            for (int side = 0; side < 2; ++side) descs[i].xmask_a[side] = g->xfuse[i].mask[side];
+    }
+    for (size_t i = 0; i < nm; ++i) {
+        const int c = chain_of[i];
+        if (c < 0) {
+            const int nt = (int)((g->members[i]->cells + per_tile - 1) / per_tile);
+            descs[i].tile_begin = (int)tiles.size();
+            descs[i].ntiles = nt;
+            tiles.insert(tiles.end(), (size_t)nt, (int)i);
+            continue;
         }
-        const int nt = (int)((m->cells + per_tile - 1) / per_tile);
-        d.tile_begin = (int)tiles.size();
-        d.ntiles = nt;
-        descs.push_back(d);
-        tiles.insert(tiles.end(), (size_t)nt, (int)i);
+        if (pos[i] != 0) continue;  // a chain's tiles go with its first member
+        // a chain tile: the same R rows of all M members, each member on
+        // ceil(R X / 64) wavefronts, M of those within the work-group and
+        // R M <= kChainRows; the most rows that fit
+        const std::vector<int> &ch = g->chains[c];
+        const int M = (int)ch.size();
+        const lbm_solver *m0 = g->members[ch[0]];
+        const int X = m0->cfg.dims[0];
+        const long long rows = (long long)m0->cfg.dims[1] * m0->cfg.dims[2];
+        auto lanes_of = [X](int r) { return (r * X + 63) / 64 * 64; };
+        int R = kChainRows / M;
+        while (R > 1 && M * lanes_of(R) > g->wg) --R;
+        XChain xc{};
+        xc.nmem = M;
+        xc.rows = R;
+        xc.nx = X;
+        xc.lanes = lanes_of(R);
+        xc.tile_begin = (int)tiles.size();
+        for (int k = 0; k < M; ++k) {
+            const int mem = ch[k];
+            xc.mem[k] = mem;
+            xc.left[k] = g->xfuse[mem].nb[0] >= 0 ? pos[g->xfuse[mem].nb[0]] : -1;
+            xc.right[k] = g->xfuse[mem].nb[1] >= 0 ? pos[g->xfuse[mem].nb[1]] : -1;
+        }
+        const int nt = (int)((rows + R - 1) / R);
+        for (int k = 0; k < M; ++k) {
+            descs[ch[k]].tile_begin = xc.tile_begin;
+            descs[ch[k]].ntiles = nt;
+        }
+        tiles.insert(tiles.end(), (size_t)nt, -(int)xch.size() - 1);
+        xch.push_back(xc);
     }
     GCHK(grow_table(g->d_tiles, g->tiles_cap, tiles.size()));
     GCHK(grow_table(g->d_descs, g->descs_cap, descs.size()));
+    if (!xch.empty()) GCHK(grow_table(g->d_chains, g->chains_cap, xch.size()));
     g->ntiles = (int)tiles.size();
     if (!tiles.empty())
         GCHK(hipMemcpyAsync(g->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, g->stream));
+    if (!xch.empty())
+        GCHK(hipMemcpyAsync(g->d_chains, xch.data(), sizeof(XChain) * xch.size(), hipMemcpyHostToDevice, g->stream));
     if (!descs.empty())
         GCHK(hipMemcpyAsync(g->d_descs, descs.data(), sizeof(DomDev) * descs.size(), hipMemcpyHostToDevice, g->stream));
     const long long ctr[2] = {g->step_counter, g->step_counter};
@@ -1430,6 +1476,7 @@ extern "C" int lbm_group_destroy(lbm_group *g) {
     }
     (void)hipFree(g->d_descs);
     (void)hipFree(g->d_tiles);
+    (void)hipFree(g->d_chains);
     (void)hipFree(g->d_ctr);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
@@ -1554,9 +1601,9 @@ extern "C" int lbm_group_set_graph(lbm_group *g, int enable) {
 
 // The copy plan of the exchange.  With fuse_x, an x-face comm between
 // equally sized members with the reference's face geometry is done by the
-// step kernel instead: the sender across the face writes the receiver's
-// ghost layer (after alpha) and inner layer (after beta) directly; its
-// copies are kept for lbm_group_sync only.
+// step kernel instead: members joined that way form chains along x whose
+// tiles hold the same rows of every member (XChain), and the face values move
+// through LDS; the copies of a joined face are kept for lbm_group_sync only.
 static bool fusable_x(const lbm_group *g, const GroupComm &c, const GroupComm &p) {
     if (!g->fuse_x || c.normal[0] == 0) return false;
     const lbm_solver *me = g->members[c.member], *nb = g->members[c.dst];
@@ -1572,40 +1619,100 @@ static bool fusable_x(const lbm_group *g, const GroupComm &c, const GroupComm &p
     return c.recv_origin[0] == X - 1 && c.send_origin[0] == X - 2 && p.send_origin[0] == 1 && p.recv_origin[0] == 0;
 }
 
+// after alpha: the slots of my ghost layer the neighbour's inner layer
+// replaces (src/CController.hpp:265-320; mode 1: those an inner cell reads)
+static uint32_t alpha_mask(const lbm_group *g, const GroupComm &c) {
+    if (g->exchange_mode != 1) return (1u << 19) - 1;
+    uint32_t m = 0;
+    for (int i = 0; i < 19; ++i)
+        if (c.normal[0] * ex(i) + c.normal[1] * ey(i) + c.normal[2] * ez(i) < 0) m |= 1u << i;
+    return m;
+}
+
+// Joins the fusable x faces into chains (lines, or rings where x is
+// periodic) and keeps those a chain tile can hold: at most kMaxChain
+// members, one row of all of them within a work-group.
+static void plan_chains(lbm_group *g, const std::vector<char> &fuse) {
+    const int nm = (int)g->members.size();
+    g->xfuse.assign((size_t)nm, lbm_group::XFuse{{-1, -1}, {0u, 0u}});
+    g->chains.clear();
+    for (size_t ci = 0; ci < g->comms.size(); ++ci) {
+        if (!fuse[ci]) continue;
+        const GroupComm &c = g->comms[ci];
+        const int side = c.normal[0] == 1 ? 0 : 1;  // inward normal +1: my x- face
+        lbm_group::XFuse &f = g->xfuse[c.member];
+        if (f.nb[side] >= 0) {  // two comms on one face: neither is joined
+            f.nb[side] = -2;
+            continue;
+        }
+        f.nb[side] = c.dst;
+        f.mask[side] = alpha_mask(g, c);
+    }
+    // a join holds only when both sides agree
+    for (int i = 0; i < nm; ++i)
+        for (int side = 0; side < 2; ++side) {
+            const int j = g->xfuse[i].nb[side];
+            if (j == -2 || (j >= 0 && g->xfuse[j].nb[1 - side] != i)) g->xfuse[i].nb[side] = -1;
+        }
+    for (int i = 0; i < nm; ++i)
+        for (int side = 0; side < 2; ++side) {
+            const int j = g->xfuse[i].nb[side];
+            if (j >= 0 && g->xfuse[j].nb[1 - side] != i) g->xfuse[i].nb[side] = -1;
+        }
+    std::vector<char> seen((size_t)nm, 0);
+    auto walk = [&](int start) {
+        std::vector<int> ch;
+        for (int cur = start; cur >= 0 && !seen[cur]; cur = g->xfuse[cur].nb[1]) {
+            seen[cur] = 1;
+            ch.push_back(cur);
+        }
+        return ch;
+    };
+    std::vector<std::vector<int>> found;
+    for (int i = 0; i < nm; ++i)  // lines start at a member with no left partner
+        if (!seen[i] && g->xfuse[i].nb[0] < 0 && g->xfuse[i].nb[1] >= 0) found.push_back(walk(i));
+    for (int i = 0; i < nm; ++i)  // what is left are rings
+        if (!seen[i] && g->xfuse[i].nb[0] >= 0) found.push_back(walk(i));
+    for (auto &ch : found) {
+        const int M = (int)ch.size();
+        const int X = g->members[ch[0]]->cfg.dims[0];
+        if (M <= kMaxChain && M * ((X + 63) / 64 * 64) <= g->wg) {
+            g->chains.push_back(ch);
+            continue;
+        }
+        for (int mem : ch) g->xfuse[mem] = lbm_group::XFuse{{-1, -1}, {0u, 0u}};
+    }
+}
+
 static int build_plan(lbm_group *g) {
     std::vector<CopyOp> ops[2][4];
-    g->xfuse.assign(g->members.size(), lbm_group::XFuse{{nullptr, nullptr}, {0u, 0u}, {0, 0}});
-    for (const GroupComm &c : g->comms) {
-        const GroupComm *p = nullptr;
+    group_drop_graph(g);  // the graph holds the old tables
+    std::vector<const GroupComm *> partner(g->comms.size(), nullptr);
+    std::vector<char> fuse(g->comms.size(), 0);
+    for (size_t ci = 0; ci < g->comms.size(); ++ci) {
+        const GroupComm &c = g->comms[ci];
         for (const GroupComm &q : g->comms)
             if (q.member == c.dst && q.dst == c.member && q.normal[0] == -c.normal[0] && q.normal[1] == -c.normal[1] &&
                 q.normal[2] == -c.normal[2]) {
-                p = &q;
+                partner[ci] = &q;
                 break;
             }
-        if (!p)
+        if (!partner[ci])
             return gfail(g, LBM_ERR_STATE,
                          "comm of member " + std::to_string(c.member) + " toward " + std::to_string(c.dst) +
                              " has no counterpart");
+        fuse[ci] = fusable_x(g, c, *partner[ci]);
+    }
+    plan_chains(g, fuse);
+    for (size_t ci = 0; ci < g->comms.size(); ++ci) {
+        const GroupComm &c = g->comms[ci];
+        const GroupComm *p = partner[ci];
         const int axis = c.normal[0] ? 0 : c.normal[1] ? 1 : 2;
         lbm_solver *me = g->members[c.member], *nb = g->members[c.dst];
         std::string why;
-        // after alpha: neighbour's inner layer -> my ghost layer (src/CController.hpp:265-320)
-        uint32_t amask = (1u << 19) - 1;
-        if (g->exchange_mode == 1) {
-            amask = 0;
-            for (int i = 0; i < 19; ++i)
-                if (c.normal[0] * ex(i) + c.normal[1] * ey(i) + c.normal[2] * ez(i) < 0) amask |= 1u << i;
-        }
-        const bool fused = fusable_x(g, c, *p);
+        const uint32_t amask = alpha_mask(g, c);
+        const bool fused = fuse[ci] && g->xfuse[c.member].nb[c.normal[0] == 1 ? 0 : 1] == c.dst;
         const int slot = fused ? 3 : axis;
-        if (fused) {
-            // the sender is the neighbour, across its face opposite to mine
-            const int side = c.normal[0] == 1 ? 1 : 0;
-            g->xfuse[c.dst].xnb[side] = me->dd;
-            g->xfuse[c.dst].xmask_a[side] = amask;
-            g->xfuse[c.member].xrecv[1 - side] = 1;
-        }
         CopyOp a;
         int rc = fill_op(nb, p->send_origin, me, c.recv_origin, c.size, amask, a, why);
         if (rc) return gfail(g, rc, why);
@@ -1670,7 +1777,7 @@ static int group_launch_half(lbm_group *g, bool beta, bool timed) {
         GCHK(hipEventRecord(ea, g->stream));
     }
     GCHK(launch_step_kernel(g->storage, beta, g->variant, g->lds, g->ntiles, g->wg, g->d_descs, g->d_tiles,
-                            g->stream, g->d_ctr));
+                            g->stream, g->d_ctr, g->d_chains));
     if (timed) {
         GCHK(hipEventRecord(eb, g->stream));
         g->events.push_back({ea, eb, beta ? 0 : 2});
@@ -1762,7 +1869,9 @@ extern "C" int lbm_group_set_launch(lbm_group *g, int work_group_size, int kerne
     g->wg = work_group_size;
     g->variant = kernel_variant;
     group_drop_graph(g);
-    return group_upload_descs(g);
+    if (g->fuse_x) g->plan_built = false;  // which chains a work-group holds depends on its size
+    g->descs_dirty = true;
+    return g->plan_built ? group_upload_descs(g) : LBM_OK;
 }
 
 extern "C" int lbm_group_get_launch(const lbm_group *g, int *work_group_size, int *kernel_variant) {
@@ -1800,6 +1909,7 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
     for (int wi = 0; wi < 3; ++wi)
         for (int vi = 0; vi < 3; ++vi) {
             int rc = lbm_group_set_launch(g, wgs[wi], vars[vi] | (g->variant & 4));
+            if (!rc) rc = group_prepare(g);
             if (rc) return rc;
             rc = group_launch_half(g, true, false);                 // warm-up pair
             if (!rc) rc = group_launch_half(g, false, false);
@@ -1825,6 +1935,7 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     int rc = lbm_group_set_launch(g, bw, bv);
+    if (!rc) rc = group_prepare(g);
     if (rc) return rc;
     if (best_wg) *best_wg = bw;
     if (best_variant) *best_variant = bv;
