@@ -566,6 +566,25 @@ __device__ __forceinline__ void cell_store(gchar *const *pl, unsigned g, int cel
     }
 }
 
+// cell_store restricted to the slots (beta: populations) k whose bit in
+// `mask` equals `bit`; the mask is wavefront-uniform, so whole-wavefront
+// stores are kept.
+template <class SP, bool BETA, bool NTS>
+__device__ __forceinline__ void cell_store_part(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
+                                                const typename SP::T *f, unsigned mask, unsigned bit) {
+    const unsigned own = cell_base(g) * SP::kBytes;
+#pragma unroll
+    for (int k = 0; k < 19; ++k) {
+        if (((mask >> k) & 1u) != bit) continue;
+        if (BETA && k < 18) {
+            const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[k], cells) : g + (unsigned)d[k];
+            kst<SP, NTS>(pl[k], cell_base(idx) * SP::kBytes, k, f[k]);
+        } else {
+            kst<SP, NTS>(pl[k], own, k, f[k]);
+        }
+    }
+}
+
 // Alpha's slot values: slot k takes f[opp(k)] where the cell collided, else
 // its loaded value f[k] (the reference stores nothing there,
 // lbm_alpha.cl:305-343; rewriting the same value keeps memory identical and
@@ -728,6 +747,13 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
     const int to = !in ? -1 : x == X - 2 ? C->right[m] : (x == 1 ? C->left[m] : -1);
     const int sside = x == X - 2 ? 0 : 1;
     T *so = xv + (((to < 0 ? 0 : to) * R + r) * 2 + sside) * 19;
+    // alpha: the slots a ghost cell of this wavefront takes from LDS wait for
+    // the barrier, the wavefront stores the rest before it (uniform per
+    // wavefront).  Beta stores everything after it: its scatter addresses
+    // would stay live across the barrier.
+    const int from = !live ? -1 : x == 0 ? C->left[m] : (x == X - 1 ? C->right[m] : -1);
+    const bool side0 = __ballot(from >= 0 && x == 0) != 0, side1 = __ballot(from >= 0 && x == X - 1) != 0;
+    const unsigned late = BETA ? (1u << 19) - 1u : (side0 ? D->xmask_a[0] : 0u) | (side1 ? D->xmask_a[1] : 0u);
     if (live) {
         flag = ((const guchar *)D->flags)[g];
         cell_load<SP, BETA, NTL>(pl, (unsigned)g, cells, d, wrap, f);
@@ -738,6 +764,7 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
             for (int k = 0; k < 19; ++k)
                 if (!BETA || ex(k) == (sside ? -1 : 1)) so[k] = f[k];
         }
+        cell_store_part<SP, BETA, NTS>(pl, (unsigned)g, cells, d, wrap, f, late, 0u);
     } else if (to >= 0) {
         // frozen sender: alpha's slot values are its own slots, beta's are the
         // slots its scatter would write
@@ -753,7 +780,6 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
     }
     __syncthreads();
     if (!live) return;
-    const int from = x == 0 ? C->left[m] : (x == X - 1 ? C->right[m] : -1);
     if (from >= 0) {
         const int side = x == 0 ? 0 : 1;
         const T *o = xv + ((m * R + r) * 2 + side) * 19;
@@ -764,7 +790,7 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
             if (take) f[k] = o[k];
         }
     }
-    cell_store<SP, BETA, NTS>(pl, (unsigned)g, cells, d, wrap, f);
+    cell_store_part<SP, BETA, NTS>(pl, (unsigned)g, cells, d, wrap, f, late, 1u);
     if (flag == kFlagGhost) return;
     store_macros<T>(D, (unsigned)g, cells, vx, vy, vz, dens);
 }
