@@ -182,7 +182,7 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=200)
     ap.add_argument("--exchange-mode", type=int, default=1)
     ap.add_argument("--wg", type=int, default=256)
-    ap.add_argument("--kernel-variant", type=int, default=0)
+    ap.add_argument("--kernel-variant", type=int, default=3)
     ap.add_argument("--lds-per-block", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--fused-x", type=int, default=1, choices=[0, 1],

@@ -547,6 +547,16 @@ __device__ __forceinline__ void cell_load(gchar *const *pl, unsigned g, int cell
     }
 }
 
+// One beta scatter store.  A wavefront's run for a population with e_x != 0
+// is shifted by one element, so its first and last cache lines are shared
+// with the neighbouring runs: those stores keep the default policy (the line
+// stays in L2 until the other part arrives); the aligned ones may stream.
+template <class SP, bool NTS>
+__device__ __forceinline__ void beta_st(gchar *base, unsigned off, int i, typename SP::T v) {
+    if (NTS && ex(i) == 0) kst<SP, true>(base, off, i, v);
+    else kst<SP, false>(base, off, i, v);
+}
+
 // Beta scatters f[i] to slot i of W(g + d_i) (lbm_beta.cl:708-784).  Alpha
 // stores every slot of its cell: f[] already holds slot k's value at k.
 template <class SP, bool BETA, bool NTS>
@@ -557,7 +567,7 @@ __device__ __forceinline__ void cell_store(gchar *const *pl, unsigned g, int cel
 #pragma unroll
         for (int i = 0; i < 18; ++i) {
             const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
-            kst<SP, NTS>(pl[i], cell_base(idx) * SP::kBytes, i, f[i]);
+            beta_st<SP, NTS>(pl[i], cell_base(idx) * SP::kBytes, i, f[i]);
         }
         kst<SP, NTS>(pl[18], own, 18, f[18]);
     } else {
@@ -578,7 +588,7 @@ __device__ __forceinline__ void cell_store_part(gchar *const *pl, unsigned g, in
         if (((mask >> k) & 1u) != bit) continue;
         if (BETA && k < 18) {
             const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[k], cells) : g + (unsigned)d[k];
-            kst<SP, NTS>(pl[k], cell_base(idx) * SP::kBytes, k, f[k]);
+            beta_st<SP, NTS>(pl[k], cell_base(idx) * SP::kBytes, k, f[k]);
         } else {
             kst<SP, NTS>(pl[k], own, k, f[k]);
         }
