@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/lbm_capi.h"
+#include "exchange_plan.hpp"
 #include "lbm_device.hpp"
 
 using namespace lbm;
@@ -189,6 +190,7 @@ struct CopyOp {
     unsigned char pops[20];
     int block_begin;
     int cells;
+    int vec;            // x-consecutive elements per work-item: 4 where both x runs are 4-aligned, else 1
     long long stop_at;  // half-step index from which the receiving domain is frozen
 };
 
@@ -197,14 +199,19 @@ struct CopyOp {
 template <unsigned B>
 __global__ void __launch_bounds__(256) exchange_kernel(const CopyOp *__restrict__ ops, int nops,
                                                        const long long *ctr) {
-    int k = 0;
-    while (k + 1 < nops && ops[k + 1].block_begin <= (int)blockIdx.x) ++k;
-    const CopyOp &op = ops[k];
+    int lo = 0, hi = nops - 1;  // the op whose blocks hold this one (block_begin ascending)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ops[mid].block_begin <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const CopyOp &op = ops[lo];
     if (ctr && *ctr - 1 >= op.stop_at) return;
     const int t = ((int)blockIdx.x - op.block_begin) * (int)blockDim.x + (int)threadIdx.x;
-    if (t >= op.cells * op.npop) return;
-    const int p = t / op.cells;
-    const int e = t - p * op.cells;
+    const int per = op.cells / op.vec;  // work-items per population
+    if (t >= per * op.npop) return;
+    const int p = t / per;
+    const int e = (t - p * per) * op.vec;
     const int bx = e % op.sz[0];
     const int r = e / op.sz[0];
     const int by = r % op.sz[1];
@@ -214,6 +221,11 @@ __global__ void __launch_bounds__(256) exchange_kernel(const CopyOp *__restrict_
     const int i = op.pops[p];
     const gchar *sp = (const gchar *)op.src + (size_t)i * kBlk * B + cell_base(gs) * B;
     gchar *dp = (gchar *)op.dst + (size_t)i * kBlk * B + cell_base(gd) * B;
+    if (op.vec == 4) {  // gs, gd multiples of 4: the four cells are contiguous in one layout block
+        typedef unsigned V __attribute__((ext_vector_type(B)));  // 4 elements of B bytes
+        *(__attribute__((address_space(1))) V *)dp = *(const __attribute__((address_space(1))) V *)sp;
+        return;
+    }
     if (B == 8)
         *(__attribute__((address_space(1))) unsigned long long *)dp =
             *(const __attribute__((address_space(1))) unsigned long long *)sp;
@@ -1057,6 +1069,9 @@ static int fill_op(lbm_solver *src, const int *so, lbm_solver *dst, const int *d
         return LBM_ERR_OUT_OF_RANGE;
     }
     op.cells = (int)cells;
+    op.vec = (so[0] % 4 == 0 && dof[0] % 4 == 0 && size[0] % 4 == 0 && op.src_nx % 4 == 0 && op.dst_nx % 4 == 0)
+                 ? 4
+                 : 1;
     op.block_begin = 0;
     op.stop_at = dst->stop_at;
     return LBM_OK;
@@ -1087,7 +1102,7 @@ extern "C" int lbm_copy_box(lbm_solver *src, const int src_origin[3], lbm_solver
     CopyOp *d_op = nullptr;
     SCHK(hipMalloc((void **)&d_op, sizeof(CopyOp)));
     SCHK(hipMemcpyAsync(d_op, &op, sizeof(CopyOp), hipMemcpyHostToDevice, s->stream));
-    SCHK(exchange_launch(s->cfg.storage, d_op, 1, (op.cells * op.npop + 255) / 256, s->stream));
+    SCHK(exchange_launch(s->cfg.storage, d_op, 1, (op.cells / op.vec * op.npop + 255) / 256, s->stream));
     SCHK(hipStreamSynchronize(s->stream));
     SCHK(hipFree(d_op));
     return LBM_OK;
@@ -1686,6 +1701,7 @@ static void plan_chains(lbm_group *g, const std::vector<char> &fuse) {
 
 static int build_plan(lbm_group *g) {
     std::vector<CopyOp> ops[2][4];
+    std::vector<HOp> hops[2][4];
     group_drop_graph(g);  // the graph holds the old tables
     std::vector<const GroupComm *> partner(g->comms.size(), nullptr);
     std::vector<char> fuse(g->comms.size(), 0);
@@ -1713,22 +1729,52 @@ static int build_plan(lbm_group *g) {
         const uint32_t amask = alpha_mask(g, c);
         const bool fused = fuse[ci] && g->xfuse[c.member].nb[c.normal[0] == 1 ? 0 : 1] == c.dst;
         const int slot = fused ? 3 : axis;
-        CopyOp a;
-        int rc = fill_op(nb, p->send_origin, me, c.recv_origin, c.size, amask, a, why);
-        if (rc) return gfail(g, rc, why);
-        if (a.npop) ops[0][slot].push_back(a);
+        HOp a{c.dst, c.member, {}, {}, {}, amask};
         // after beta: neighbour's ghost layer -> my inner layer, n.e > 0 only (src/CController.hpp:322-383)
-        CopyOp b;
-        rc = fill_op(nb, p->recv_origin, me, c.send_origin, c.size, norm_mask(c.normal), b, why);
+        HOp b{c.dst, c.member, {}, {}, {}, norm_mask(c.normal)};
+        for (int ax = 0; ax < 3; ++ax) {
+            a.so[ax] = p->send_origin[ax];
+            a.dof[ax] = c.recv_origin[ax];
+            b.so[ax] = p->recv_origin[ax];
+            b.dof[ax] = c.send_origin[ax];
+            a.sz[ax] = b.sz[ax] = c.size[ax];
+        }
+        // each box is checked against both subdomains before any merging
+        CopyOp chk;
+        int rc = fill_op(nb, a.so, me, a.dof, a.sz, a.mask, chk, why);
+        if (!rc) rc = fill_op(nb, b.so, me, b.dof, b.sz, b.mask, chk, why);
         if (rc) return gfail(g, rc, why);
-        if (b.npop) ops[1][slot].push_back(b);
+        hops[0][slot].push_back(a);
+        hops[1][slot].push_back(b);
     }
+    // One copy launch per half-step instead of one per axis where the order
+    // resolves on the host.  Not with frozen members: a frozen receiver skips
+    // its copies, which the resolved plan would read through.
+    bool frozen = false;
+    for (const lbm_solver *m : g->members) frozen |= m->stop_at != std::numeric_limits<long long>::max();
+    for (int k = 0; k < 2 && !frozen; ++k) {
+        std::vector<HOp> merged;
+        if (!merge_phases(hops[k], merged)) continue;
+        hops[k][0].swap(merged);
+        hops[k][1].clear();
+        hops[k][2].clear();
+    }
+    for (int k = 0; k < 2; ++k)
+        for (int q = 0; q < 4; ++q)
+            for (const HOp &h : hops[k][q]) {
+                if (!h.mask) continue;
+                CopyOp op;
+                std::string why;
+                const int rc = fill_op(g->members[h.sm], h.so, g->members[h.dm], h.dof, h.sz, h.mask, op, why);
+                if (rc) return gfail(g, rc, why);
+                ops[k][q].push_back(op);
+            }
     for (int k = 0; k < 2; ++k)
         for (int p = 0; p < 4; ++p) {
             int blocks = 0;
             for (auto &op : ops[k][p]) {
                 op.block_begin = blocks;
-                blocks += (op.cells * op.npop + 255) / 256;
+                blocks += (op.cells / op.vec * op.npop + 255) / 256;
             }
             g->nops[k][p] = (int)ops[k][p].size();
             g->nblocks[k][p] = blocks;
