@@ -218,9 +218,10 @@ int lbm_group_get_timing(lbm_group *g, double *step_kernel_ms, long long *step_k
 int lbm_group_set_launch(lbm_group *g, int work_group_size, int kernel_variant);
 int lbm_group_get_launch(const lbm_group *g, int *work_group_size, int *kernel_variant);
 /* autotuner (new): times `pairs` step pairs for each work-group size in
- * {64,128,256} x cells-per-lane in {1,2,4}, keeps the fastest; the timed steps
- * are real steps of the simulation (18 * (pairs + 1) of them, plus one if the
- * counter was odd) */
+ * {64,128,256} x kernel variant in {0,3,8,16} (1 cell per lane with the
+ * default or the non-temporal cache policy, 2 or 4 cells per lane), keeps the
+ * fastest; the timed steps are real steps of the simulation (24 * (pairs + 1)
+ * of them, plus one if the counter was odd) */
 int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step);
 /* the same averages split into beta and alpha step kernels */
 int lbm_group_get_timing_detail(lbm_group *g, double *beta_ms, double *alpha_ms, double *exchange_ms);

@@ -66,10 +66,13 @@ def run_batched(cases: Sequence[Cavity], steps: int, *, autotune_pairs: int = 2,
     t0 = time.perf_counter()
     tuned = None
     done = 0
-    if autotune_pairs > 0 and steps >= 18 * (autotune_pairs + 1):
+    # the tuner runs 12 launch shapes x (pairs + 1) step pairs, plus one step
+    # if the counter is odd; those are real steps of the run
+    if autotune_pairs > 0 and steps >= 24 * (autotune_pairs + 1) + 1:
+        before = solvers[0].step_counter
         tuned = g.autotune(autotune_pairs)
-        done = 18 * (autotune_pairs + 1)
         g.wait()
+        done = solvers[0].step_counter - before
     t1 = time.perf_counter()
     g.step(steps - done)
     g.wait()

@@ -1928,8 +1928,9 @@ extern "C" int lbm_group_get_launch(const lbm_group *g, int *work_group_size, in
 }
 
 // Autotuner: times `pairs` beta+alpha step pairs (with their exchanges) for
-// every work-group size in {64, 128, 256} and every variant in {0, 8, 16}
-// (1, 2, 4 cells per lane) and keeps the fastest.  The steps it runs are
+// every work-group size in {64, 128, 256} and every variant in {0, 3, 8, 16}
+// (1 cell per lane with the default or the non-temporal cache policy, 2 or 4
+// cells per lane) and keeps the fastest.  The steps it runs are
 // ordinary simulation steps: every launch shape computes the same
 // arithmetic, so the state is the one the same number of steps gives with
 // any shape.
@@ -1946,14 +1947,14 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
     }
     if (g->ntiles == 0) return gfail(g, LBM_ERR_STATE, "autotune needs at least one member");
     const int wgs[3] = {64, 128, 256};
-    const int vars[3] = {0, 8, 16};
+    const int vars[4] = {0, 3, 8, 16};
     double best = 1e300;
     int bw = g->wg, bv = g->variant;
     hipEvent_t e0, e1;
     GCHK(hipEventCreate(&e0));
     GCHK(hipEventCreate(&e1));
     for (int wi = 0; wi < 3; ++wi)
-        for (int vi = 0; vi < 3; ++vi) {
+        for (int vi = 0; vi < 4; ++vi) {
             int rc = lbm_group_set_launch(g, wgs[wi], vars[vi] | (g->variant & 4));
             if (!rc) rc = group_prepare(g);
             if (rc) return rc;
