@@ -306,6 +306,8 @@ def main(argv=None):
     ap.add_argument("--edge-max", type=int, default=192)
     ap.add_argument("--steps-max", type=int, default=5000)
     ap.add_argument("--mode", choices=["mixed", "compare"], default="compare")
+    ap.add_argument("--kernel-variant", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true", help="tune each storage pool's launch shape once")
     a = ap.parse_args(argv)
 
     def log(m):
@@ -317,9 +319,11 @@ def main(argv=None):
         reqs = request_stream(a.requests, rate, a.seed, edge_range=(32, a.edge_max), steps_range=(500, a.steps_max))
         log(f"rate {rate}: {len(reqs)} requests")
         if a.mode == "compare":
-            report["rates"][str(rate)] = compare(reqs, a.budget_gb * 1e9, chunk_ms=a.chunk_ms, log=log)
+            report["rates"][str(rate)] = compare(reqs, a.budget_gb * 1e9, chunk_ms=a.chunk_ms,
+                                                 kernel_variant=a.kernel_variant, autotune=a.autotune, log=log)
         else:
-            with Server(a.budget_gb * 1e9, chunk_ms=a.chunk_ms) as srv:
+            with Server(a.budget_gb * 1e9, chunk_ms=a.chunk_ms, kernel_variant=a.kernel_variant,
+                        autotune=a.autotune) as srv:
                 report["rates"][str(rate)] = serve(srv, reqs).summary()
     print(json.dumps(report))
 
