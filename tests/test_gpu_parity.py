@@ -336,8 +336,10 @@ def test_fused_x_faces_match_copy_exchange(mode, domain_size, nums):
 @pytest.mark.parametrize("wg", [64, 128, 256])
 @pytest.mark.parametrize("domain_size", [(56, 12, 10), (200, 12, 10), (260, 12, 10)])
 def test_fused_x_chain_fits_work_group_or_copies(wg, domain_size):
-    """A chain tile holds one row of every member; chains wider than the
-    work-group keep the copy exchange.  Results never depend on which."""
+    """A chain tile holds whole rows of every member, or for wider rows the
+    64-cell runs next to the faces (the interior runs in tiles of their own);
+    chains whose face runs do not fit the work-group keep the copy exchange.
+    Results never depend on which."""
     nums = (4, 1, 1) if domain_size[0] == 56 else (2, 1, 1)
     say(f"case: fused x chain {domain_size} {nums} wg {wg}")
     fused_vs_copy(domain_size, nums, 1, wg=wg, step_runs=(3, 4))
@@ -350,11 +352,14 @@ def test_fused_x_periodic_ring_of_two():
     fused_vs_copy((36, 14, 12), (2, 1, 1), 1, periodic=(True, False, False), step_runs=(5, 6))
 
 
-def test_fused_x_with_frozen_member():
-    """A member frozen at its own step count inside a chain: the frozen
-    sender's slots as they stand feed its partner, as the copies do."""
-    say("case: fused x chain with a frozen member")
-    fused_vs_copy((42, 12, 10), (3, 1, 1), 1, stops=((1, 3), (2, 6)), step_runs=(5, 6))
+@pytest.mark.parametrize("domain_size,nums", [((42, 12, 10), (3, 1, 1)), ((260, 10, 8), (2, 1, 1))])
+def test_fused_x_with_frozen_member(domain_size, nums):
+    """A member frozen at its own step count inside a chain (whole-row and
+    one-row tiles): the frozen sender's slots as they stand feed its partner,
+    as the copies do."""
+    say(f"case: fused x chain {domain_size} {nums} with a frozen member")
+    stops = ((1, 3), (2, 6)) if nums[0] == 3 else ((1, 3),)
+    fused_vs_copy(domain_size, nums, 1, stops=stops, step_runs=(5, 6))
 
 
 def test_group_2x2x2_fused_matches_oracle_decomposition():
@@ -378,15 +383,17 @@ def test_group_2x2x2_fused_matches_oracle_decomposition():
         s.close()
 
 
-def test_fused_x_periodic_channel():
-    """A periodic channel (self comms across x) with fused x faces: FLUID
-    slots equal to the copy-based exchange after 17 steps."""
-    say("case: fused x periodic channel")
+@pytest.mark.parametrize("dims", [(24, 18, 16), (300, 10, 8)])
+def test_fused_x_periodic_channel(dims):
+    """A periodic channel (self comms across x) with fused x faces, a ring of
+    one (whole rows, and one-row tiles with interior tiles when X = 300):
+    FLUID slots equal to the copy-based exchange after 17 steps."""
+    say(f"case: fused x periodic channel {dims}")
     from turbulent_lbm_multigpu_amd import precision
 
     out = []
     for fused in (False, True):
-        s, g, _ = precision.build_channel((24, 18, 16), _capi.STORAGE_F32, tau=0.55, gx=2e-5, fill=0.04, rmin=2.0,
+        s, g, _ = precision.build_channel(dims, _capi.STORAGE_F32, tau=0.55, gx=2e-5, fill=0.04, rmin=2.0,
                                           rmax=3.0, amp=1e-3, seed=9)
         g.set_fused_x(fused)
         g.step(17)

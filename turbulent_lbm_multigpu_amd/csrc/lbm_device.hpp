@@ -256,7 +256,10 @@ constexpr int kMaxChain = 8;    // members of one chain
 constexpr int kChainRows = 16;  // members x rows of one chain tile (LDS capacity)
 struct XChain {
     int nmem, rows, nx, tile_begin;
-    int lanes;             // work-items per member: rows * nx rounded up to whole wavefronts
+    int lanes;             // whole-row tiles: work-items per member, rows * nx rounded up to whole wavefronts
+    int nedge;             // one-row tiles: 64-cell runs holding columns 0, 1, X-2, X-1 (else 0)
+    int edge[3];           //   their run indices
+    int ninner, ipasses;   //   interior runs 1 .. ninner per member, in ipasses tiles after each row's (else 0)
     int mem[kMaxChain];    // descriptor index of each position
     int left[kMaxChain];   // position whose x = X-2 column feeds my x = 0 ghost layer, or -1
     int right[kMaxChain];  // position whose x = 1 column feeds my x = X-1 ghost layer, or -1
@@ -701,33 +704,79 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
         if (flag[j] != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx[j], vy[j], vz[j], dn[j]);
 }
 
-// One chain tile: position m of the chain, row r of the tile and column x
-// per work-item (whole rows, so the x = X-2 / x = 1 columns feeding a ghost
-// layer are in this work-group).  Senders put their face values in LDS; after
-// the barrier a ghost cell of a joined face takes them in place of its own:
+// An interior tile of a one-row chain: round `pass` of the interior runs of
+// row `row` (columns 64 .. X-3 of every member, one 64-cell run per
+// wavefront).  No column of a joined face: no LDS, no barrier.
+template <class SP, bool BETA, bool NTS, bool NTL>
+__device__ __forceinline__ void chain_interior(const DomDev *__restrict__ doms, const XChain *__restrict__ C, int row,
+                                               int pass, long long h) {
+    const int X = C->nx, inner = C->ninner;
+    const int u = pass * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    if (u >= C->nmem * inner) return;  // uniform per wavefront
+    const int m = __builtin_amdgcn_readfirstlane(u / inner);
+    const DomDev *D = doms + C->mem[m];
+    if (h >= D->stop_at) return;
+    const int cells = (int)D->cells;
+    const int gs = row * X + 64 * (1 + u - m * inner);
+    gchar *pl[19];
+    {
+        gchar *base = (gchar *)D->dd;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) pl[i] = base + (size_t)i * kBlk * SP::kBytes;
+    }
+    int d[19];
+    bool wrap = false;
+    if (BETA) {
+        const int XY = X * D->ny;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) d[i] = ex(i) + X * ey(i) + XY * ez(i);
+        const int dmax = 1 + X + XY;
+        wrap = gs < dmax || gs + 64 + dmax > cells;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) d[i] = 0;
+    }
+    cell_scalar<SP, BETA, NTS, NTL>(pl, (unsigned)(gs + (int)(threadIdx.x & 63)), cells, d, wrap, D);
+}
+
+// One chain tile.  Senders put their face values in LDS; after the barrier a
+// ghost cell of a joined face takes them in place of its own:
 //   alpha: the slots its receiver mask names (the sender's slot values after
 //          alpha, src/CController.hpp:265-320);
 //   beta:  the populations it scatters into the inner layer, n.e > 0 (the
 //          sender's values scattered into its own ghost layer,
 //          src/CController.hpp:322-383).
 // A frozen member stores nothing; as a sender it feeds its slots as they
-// stand, which is what the copy exchange reads from it.  Each member has its
-// own whole wavefronts, so the member and every address base derived from it
-// stay uniform per wavefront.
+// stand, which is what the copy exchange reads from it.  Two shapes:
+//   whole rows (nedge == 0): R rows of every member, `lanes` work-items per
+//     member (position m, row r, column x per work-item);
+//   one row in 64-cell runs (nedge > 0): the runs holding columns 0, 1, X-2
+//     and X-1 of every member, with the handoff; the interior runs follow in
+//     `ipasses` tiles of their own (chain_interior).
+// Each wavefront holds one member, so the member and every address base
+// derived from it stay uniform per wavefront.
 template <class SP, bool BETA, bool NTS, bool NTL>
-__device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, const XChain *__restrict__ C, int t,
+__device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, const XChain *__restrict__ C, int rt,
                                            long long h) {
     using T = typename SP::T;
     __shared__ T xv[kChainRows * 2 * 19];
     const int X = C->nx, R = C->rows, span = R * X;
-    const int q = (int)threadIdx.x;
-    const int m = __builtin_amdgcn_readfirstlane(q / C->lanes);
-    const int r = (q - m * C->lanes) / X;
-    const int x = q - m * C->lanes - r * X;
-    const bool lane = m < C->nmem && r < R;
+    const int q = (int)threadIdx.x, w = q >> 6;
+    int mm, r, x;
+    if (C->nedge == 0) {
+        mm = q / C->lanes;
+        r = (q - mm * C->lanes) / X;
+        x = q - mm * C->lanes - r * X;
+    } else {
+        mm = w / C->nedge;
+        r = 0;
+        x = 64 * C->edge[w - mm * C->nedge] + (q & 63);
+    }
+    const int m = __builtin_amdgcn_readfirstlane(mm);
+    const bool lane = m < C->nmem && r < R && x < X;
     const DomDev *D = doms + C->mem[m < C->nmem ? m : 0];
     const int cells = (int)D->cells;
-    const int gfirst = (t - C->tile_begin) * span;  // first cell of the tile's rows (equal dims along the chain)
+    const int gfirst = rt * span;  // first cell of the tile's rows (equal dims along the chain)
     const int g = gfirst + r * X + x;
     const bool in = lane && g < cells;
     const bool live = in && h < D->stop_at;
@@ -742,11 +791,11 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
     }
     int d[19];
     bool wrap = false;
+    const int dmax = 1 + X + X * D->ny;
     if (BETA) {
         const int XY = X * D->ny;
 #pragma unroll
         for (int i = 0; i < 19; ++i) d[i] = ex(i) + X * ey(i) + XY * ez(i);
-        const int dmax = 1 + X + XY;
         wrap = gfirst < dmax || gfirst + span + dmax > cells;
     } else {
 #pragma unroll
@@ -789,20 +838,20 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
         }
     }
     __syncthreads();
-    if (!live) return;
-    if (from >= 0) {
-        const int side = x == 0 ? 0 : 1;
-        const T *o = xv + ((m * R + r) * 2 + side) * 19;
-        const unsigned mask = D->xmask_a[side];
+    if (live) {
+        if (from >= 0) {
+            const int side = x == 0 ? 0 : 1;
+            const T *o = xv + ((m * R + r) * 2 + side) * 19;
+            const unsigned mask = D->xmask_a[side];
 #pragma unroll
-        for (int k = 0; k < 19; ++k) {
-            const bool take = BETA ? ex(k) == (side ? -1 : 1) : ((mask >> k) & 1u) != 0;
-            if (take) f[k] = o[k];
+            for (int k = 0; k < 19; ++k) {
+                const bool take = BETA ? ex(k) == (side ? -1 : 1) : ((mask >> k) & 1u) != 0;
+                if (take) f[k] = o[k];
+            }
         }
+        cell_store_part<SP, BETA, NTS>(pl, (unsigned)g, cells, d, wrap, f, late, 1u);
+        if (flag != kFlagGhost) store_macros<T>(D, (unsigned)g, cells, vx, vy, vz, dens);
     }
-    cell_store_part<SP, BETA, NTS>(pl, (unsigned)g, cells, d, wrap, f, late, 1u);
-    if (flag == kFlagGhost) return;
-    store_macros<T>(D, (unsigned)g, cells, vx, vy, vz, dens);
 }
 
 template <int ST, bool BETA, int VAR = 0>
@@ -825,7 +874,12 @@ __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ do
     const int t = xcd_map ? xcd_tile((int)blockIdx.x, nb) : (int)blockIdx.x;
     const int td = tile_dom[t];
     if (td < 0) {  // a tile of fused x chain -td-1 (uniform per work-group: the barrier is reached by all)
-        chain_tile<SP, BETA, NTS, NTL>(doms, chains + (-td - 1), t, h);
+        // per row tile: the handoff tile, then the interior tiles of a one-row chain
+        const XChain *C = chains + (-td - 1);
+        const int per = 1 + C->ipasses, k = t - C->tile_begin;
+        const int rt = k / per, part = k - rt * per;
+        if (part == 0) chain_tile<SP, BETA, NTS, NTL>(doms, C, rt, h);
+        else chain_interior<SP, BETA, NTS, NTL>(doms, C, rt, part - 1, h);
         return;
     }
     const DomDev *D = doms + td;

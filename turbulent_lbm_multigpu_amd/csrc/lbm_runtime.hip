@@ -1310,6 +1310,18 @@ static hipError_t grow_table(T *&ptr, size_t &cap, size_t n) {
     return e;
 }
 
+// 64-cell runs of an x row that hold columns 0, 1, X-2 and X-1 (distinct).
+static int chain_edge_runs(int X, int *edge) {
+    int n = 0;
+    const int cand[3] = {0, (X - 2) / 64, (X - 1) / 64};
+    for (int c : cand) {
+        bool seen = false;
+        for (int k = 0; k < n; ++k) seen |= edge[k] == c;
+        if (!seen) edge[n++] = c;
+    }
+    return n;
+}
+
 // Descriptor and tile tables of the fused launch for the group's current
 // members and launch shape (work-group size and cells per lane); also
 // re-publishes the host half-step count to the device counter.
@@ -1352,13 +1364,22 @@ static int group_upload_descs(lbm_group *g) {
         const int X = m0->cfg.dims[0];
         const long long rows = (long long)m0->cfg.dims[1] * m0->cfg.dims[2];
         auto lanes_of = [X](int r) { return (r * X + 63) / 64 * 64; };
-        int R = kChainRows / M;
-        while (R > 1 && M * lanes_of(R) > g->wg) --R;
         XChain xc{};
         xc.nmem = M;
-        xc.rows = R;
         xc.nx = X;
-        xc.lanes = lanes_of(R);
+        if (M * lanes_of(1) <= g->wg) {  // whole rows
+            int R = kChainRows / M;
+            while (R > 1 && M * lanes_of(R) > g->wg) --R;
+            xc.rows = R;
+            xc.lanes = lanes_of(R);
+        } else {  // one row in 64-cell runs: the face runs first, then the interior ones
+            xc.rows = 1;
+            xc.nedge = chain_edge_runs(X, xc.edge);
+            xc.ninner = std::max(0, (X - 2) / 64 - 1);
+            const int waves = g->wg / 64;
+            xc.ipasses = (M * xc.ninner + waves - 1) / waves;
+        }
+        const int R = xc.rows;
         xc.tile_begin = (int)tiles.size();
         for (int k = 0; k < M; ++k) {
             const int mem = ch[k];
@@ -1366,7 +1387,7 @@ static int group_upload_descs(lbm_group *g) {
             xc.left[k] = g->xfuse[mem].nb[0] >= 0 ? pos[g->xfuse[mem].nb[0]] : -1;
             xc.right[k] = g->xfuse[mem].nb[1] >= 0 ? pos[g->xfuse[mem].nb[1]] : -1;
         }
-        const int nt = (int)((rows + R - 1) / R);
+        const int nt = (int)((rows + R - 1) / R) * (1 + xc.ipasses);
         for (int k = 0; k < M; ++k) {
             descs[ch[k]].tile_begin = xc.tile_begin;
             descs[ch[k]].ntiles = nt;
@@ -1646,7 +1667,8 @@ static uint32_t alpha_mask(const lbm_group *g, const GroupComm &c) {
 
 // Joins the fusable x faces into chains (lines, or rings where x is
 // periodic) and keeps those a chain tile can hold: at most kMaxChain
-// members, one row of all of them within a work-group.
+// members, and either whole rows of all of them within a work-group or, for
+// wider rows, the 64-cell runs next to the faces of all of them.
 static void plan_chains(lbm_group *g, const std::vector<char> &fuse) {
     const int nm = (int)g->members.size();
     g->xfuse.assign((size_t)nm, lbm_group::XFuse{{-1, -1}, {0u, 0u}});
@@ -1691,7 +1713,10 @@ static void plan_chains(lbm_group *g, const std::vector<char> &fuse) {
     for (auto &ch : found) {
         const int M = (int)ch.size();
         const int X = g->members[ch[0]]->cfg.dims[0];
-        if (M <= kMaxChain && M * ((X + 63) / 64 * 64) <= g->wg) {
+        int edge[3];
+        const bool rows_fit = M * ((X + 63) / 64 * 64) <= g->wg;
+        const bool runs_fit = M * chain_edge_runs(X, edge) <= g->wg / 64;
+        if (M <= kMaxChain && (rows_fit || runs_fit)) {
             g->chains.push_back(ch);
             continue;
         }
