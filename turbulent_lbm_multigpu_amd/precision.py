@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--dims", type=int, nargs=3, default=[384, 192, 192])
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--storages", nargs="+", default=["f64", "f32", "f16", "bf16"])
-    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=3, help="kernel variant (3: non-temporal loads and stores)")
     a = ap.parse_args()
     res = run(tuple(a.dims), a.steps, tuple(a.storages), kernel_variant=a.variant)
     print(json.dumps({"workload": f"LES channel {a.dims[0]}x{a.dims[1]}x{a.dims[2]}, periodic x, g_x 1e-6, "
