@@ -299,10 +299,12 @@ def fluid_slots(s: LbmSolver):
     return gpu_dd(s).reshape(19, -1)[:, fl == orc.FLAG_FLUID]
 
 
-def fused_vs_copy(domain_size, nums, mode, *, wg=256, periodic=(False, False, False), stops=(), step_runs=(5, 8)):
+def fused_vs_copy(domain_size, nums, mode, *, wg=256, periodic=(False, False, False), stops=(), step_runs=(5, 8),
+                  storage=_capi.STORAGE_F32, variant=0):
     """Two identical groups, one with x faces inside the step kernel: every
     slot of every subdomain is bit-identical after each run of steps."""
-    kw = dict(inv_tau=1 / 0.52, tau=0.52, smagorinsky_cs=0.17, work_group_size=wg)
+    kw = dict(inv_tau=1 / 0.52, tau=0.52, smagorinsky_cs=0.17, work_group_size=wg, storage=storage,
+              kernel_variant=variant)
     ref, _, _ = build_gpu_group(domain_size, nums, lid=0.08, amp=1e-3, seed=11, mode=mode, periodic=periodic, **kw)
     fus, _, _ = build_gpu_group(domain_size, nums, lid=0.08, amp=1e-3, seed=11, mode=mode, periodic=periodic, **kw)
     fus.set_fused_x(True)
@@ -343,6 +345,15 @@ def test_fused_x_chain_fits_work_group_or_copies(wg, domain_size):
     nums = (4, 1, 1) if domain_size[0] == 56 else (2, 1, 1)
     say(f"case: fused x chain {domain_size} {nums} wg {wg}")
     fused_vs_copy(domain_size, nums, 1, wg=wg, step_runs=(3, 4))
+
+
+@pytest.mark.parametrize("storage", [_capi.STORAGE_F64, _capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
+@pytest.mark.parametrize("variant", [0, 3])
+def test_fused_x_storages_and_cache_policy(storage, variant):
+    """Chain tiles in every storage type, with the default and the
+    non-temporal cache policy: all slots equal to the copy exchange."""
+    say(f"case: fused x 2x2x2 storage {storage} variant {variant}")
+    fused_vs_copy((32, 28, 24), (2, 2, 2), 1, storage=storage, variant=variant, step_runs=(3, 4))
 
 
 def test_fused_x_periodic_ring_of_two():
