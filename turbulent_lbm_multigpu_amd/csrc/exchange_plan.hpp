@@ -46,7 +46,7 @@ inline void box_minus(const int *ao, const int *as, const int *io, const int *is
     }
 }
 
-// o restricted to the destination box (o_, s_) / the source box (o_, s_)
+// o restricted to the sub-box (origin bo, size bs) of its destination / of its source
 inline HOp op_at_dst(const HOp &o, const int *bo, const int *bs) {
     HOp r = o;
     for (int a = 0; a < 3; ++a) {

@@ -259,7 +259,7 @@ struct XChain {
     int lanes;             // whole-row tiles: work-items per member, rows * nx rounded up to whole wavefronts
     int nedge;             // one-row tiles: 64-cell runs holding columns 0, 1, X-2, X-1 (else 0)
     int edge[3];           //   their run indices
-    int ninner, ipasses;   //   interior runs 1 .. ninner per member, in ipasses tiles after each row's (else 0)
+    int ninner, ipasses;   //   interior runs 1 .. ninner per member, in ipasses tiles after each row's handoff tile
     int mem[kMaxChain];    // descriptor index of each position
     int left[kMaxChain];   // position whose x = X-2 column feeds my x = 0 ghost layer, or -1
     int right[kMaxChain];  // position whose x = 1 column feeds my x = X-1 ghost layer, or -1
@@ -704,8 +704,8 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
         if (flag[j] != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx[j], vy[j], vz[j], dn[j]);
 }
 
-// An interior tile of a one-row chain: round `pass` of the interior runs of
-// row `row` (columns 64 .. X-3 of every member, one 64-cell run per
+// An interior tile of a one-row chain: the `pass`-th share of the interior
+// runs of row `row` (columns 64 .. X-3 of every member, one 64-cell run per
 // wavefront).  No column of a joined face: no LDS, no barrier.
 template <class SP, bool BETA, bool NTS, bool NTL>
 __device__ __forceinline__ void chain_interior(const DomDev *__restrict__ doms, const XChain *__restrict__ C, int row,
