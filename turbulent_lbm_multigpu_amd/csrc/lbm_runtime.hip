@@ -196,16 +196,12 @@ struct CopyOp {
 
 // ctr: the group's half-step counter slot that the kernel just run has
 // written (h + 1), or null outside a counted group step.
+// ops: nops copies, followed (when nops > 1) by the op index of every block.
 template <unsigned B>
 __global__ void __launch_bounds__(256) exchange_kernel(const CopyOp *__restrict__ ops, int nops,
                                                        const long long *ctr) {
-    int lo = 0, hi = nops - 1;  // the op whose blocks hold this one (block_begin ascending)
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (ops[mid].block_begin <= (int)blockIdx.x) lo = mid;
-        else hi = mid - 1;
-    }
-    const CopyOp &op = ops[lo];
+    const int k = nops == 1 ? 0 : ((const int *)(ops + nops))[blockIdx.x];
+    const CopyOp &op = ops[k];
     if (ctr && *ctr - 1 >= op.stop_at) return;
     const int t = ((int)blockIdx.x - op.block_begin) * (int)blockDim.x + (int)threadIdx.x;
     const int per = op.cells / op.vec;  // work-items per population
@@ -1794,18 +1790,27 @@ static int build_plan(lbm_group *g) {
                 if (rc) return gfail(g, rc, why);
                 ops[k][q].push_back(op);
             }
+    // each table: the ops, then the op index of every block (exchange_kernel)
+    std::vector<int> block_op[2][4];
     for (int k = 0; k < 2; ++k)
         for (int p = 0; p < 4; ++p) {
             int blocks = 0;
-            for (auto &op : ops[k][p]) {
+            for (size_t i = 0; i < ops[k][p].size(); ++i) {
+                CopyOp &op = ops[k][p][i];
                 op.block_begin = blocks;
-                blocks += (op.cells / op.vec * op.npop + 255) / 256;
+                const int nb = (op.cells / op.vec * op.npop + 255) / 256;
+                block_op[k][p].insert(block_op[k][p].end(), (size_t)nb, (int)i);
+                blocks += nb;
             }
             g->nops[k][p] = (int)ops[k][p].size();
             g->nblocks[k][p] = blocks;
             if (ops[k][p].empty()) continue;
-            GCHK(grow_table(g->d_ops[k][p], g->ops_cap[k][p], ops[k][p].size()));
-            GCHK(hipMemcpyAsync(g->d_ops[k][p], ops[k][p].data(), sizeof(CopyOp) * ops[k][p].size(),
+            const size_t n = ops[k][p].size();
+            const size_t extra = (sizeof(int) * (size_t)blocks + sizeof(CopyOp) - 1) / sizeof(CopyOp);
+            GCHK(grow_table(g->d_ops[k][p], g->ops_cap[k][p], n + extra));
+            GCHK(hipMemcpyAsync(g->d_ops[k][p], ops[k][p].data(), sizeof(CopyOp) * n, hipMemcpyHostToDevice,
+                                g->stream));
+            GCHK(hipMemcpyAsync(g->d_ops[k][p] + n, block_op[k][p].data(), sizeof(int) * (size_t)blocks,
                                 hipMemcpyHostToDevice, g->stream));
         }
     GCHK(hipStreamSynchronize(g->stream));
