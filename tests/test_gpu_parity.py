@@ -394,10 +394,10 @@ def test_group_2x2x2_fused_matches_oracle_decomposition():
         s.close()
 
 
-@pytest.mark.parametrize("dims", [(24, 18, 16), (300, 10, 8)])
+@pytest.mark.parametrize("dims", [(24, 18, 16), (300, 9, 7)])
 def test_fused_x_periodic_channel(dims):
     """A periodic channel (self comms across x) with fused x faces, a ring of
-    one (whole rows, and one-row tiles with interior tiles when X = 300):
+    one (whole rows, and two-row run tiles with interior tiles when X = 300, the last row tile half full):
     FLUID slots equal to the copy-based exchange after 17 steps."""
     say(f"case: fused x periodic channel {dims}")
     from turbulent_lbm_multigpu_amd import precision

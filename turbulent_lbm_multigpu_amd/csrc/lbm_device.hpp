@@ -257,9 +257,9 @@ constexpr int kChainRows = 16;  // members x rows of one chain tile (LDS capacit
 struct XChain {
     int nmem, rows, nx, tile_begin;
     int lanes;             // whole-row tiles: work-items per member, rows * nx rounded up to whole wavefronts
-    int nedge;             // one-row tiles: 64-cell runs holding columns 0, 1, X-2, X-1 (else 0)
+    int nedge;             // run tiles: 64-cell runs holding columns 0, 1, X-2, X-1 (else 0)
     int edge[3];           //   their run indices
-    int ninner, ipasses;   //   interior runs 1 .. ninner per member, in ipasses tiles after each row's handoff tile
+    int ninner, ipasses;   //   interior runs 1 .. ninner per member and row, in ipasses tiles after each handoff tile
     int mem[kMaxChain];    // descriptor index of each position
     int left[kMaxChain];   // position whose x = X-2 column feeds my x = 0 ghost layer, or -1
     int right[kMaxChain];  // position whose x = 1 column feeds my x = X-1 ghost layer, or -1
@@ -704,20 +704,23 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
         if (flag[j] != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx[j], vy[j], vz[j], dn[j]);
 }
 
-// An interior tile of a one-row chain: the `pass`-th share of the interior
-// runs of row `row` (columns 64 .. X-3 of every member, one 64-cell run per
-// wavefront).  No column of a joined face: no LDS, no barrier.
+// An interior tile of a run-tiled chain: the `pass`-th share of the interior
+// runs of the rows of row tile `rt` (columns 64 .. X-3 of every member, one
+// 64-cell run per wavefront).  No column of a joined face: no LDS, no
+// barrier.
 template <class SP, bool BETA, bool NTS, bool NTL>
-__device__ __forceinline__ void chain_interior(const DomDev *__restrict__ doms, const XChain *__restrict__ C, int row,
+__device__ __forceinline__ void chain_interior(const DomDev *__restrict__ doms, const XChain *__restrict__ C, int rt,
                                                int pass, long long h) {
-    const int X = C->nx, inner = C->ninner;
+    const int X = C->nx, inner = C->ninner, per_row = C->nmem * inner;
     const int u = pass * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
-    if (u >= C->nmem * inner) return;  // uniform per wavefront
-    const int m = __builtin_amdgcn_readfirstlane(u / inner);
+    if (u >= per_row * C->rows) return;  // uniform per wavefront
+    const int r = u / per_row;
+    const int m = __builtin_amdgcn_readfirstlane((u - r * per_row) / inner);
     const DomDev *D = doms + C->mem[m];
     if (h >= D->stop_at) return;
     const int cells = (int)D->cells;
-    const int gs = row * X + 64 * (1 + u - m * inner);
+    const int gs = (rt * C->rows + r) * X + 64 * (1 + u - r * per_row - m * inner);
+    if (gs >= cells) return;  // rows past the last of the domain
     gchar *pl[19];
     {
         gchar *base = (gchar *)D->dd;
@@ -750,9 +753,9 @@ __device__ __forceinline__ void chain_interior(const DomDev *__restrict__ doms, 
 // stand, which is what the copy exchange reads from it.  Two shapes:
 //   whole rows (nedge == 0): R rows of every member, `lanes` work-items per
 //     member (position m, row r, column x per work-item);
-//   one row in 64-cell runs (nedge > 0): the runs holding columns 0, 1, X-2
-//     and X-1 of every member, with the handoff; the interior runs follow in
-//     `ipasses` tiles of their own (chain_interior).
+//   R rows in 64-cell runs (nedge > 0): the runs holding columns 0, 1, X-2
+//     and X-1 of every member and row, with the handoff; the interior runs of
+//     those rows follow in `ipasses` tiles of their own (chain_interior).
 // Each wavefront holds one member, so the member and every address base
 // derived from it stay uniform per wavefront.
 template <class SP, bool BETA, bool NTS, bool NTL>
@@ -768,9 +771,10 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
         r = (q - mm * C->lanes) / X;
         x = q - mm * C->lanes - r * X;
     } else {
-        mm = w / C->nedge;
-        r = 0;
-        x = 64 * C->edge[w - mm * C->nedge] + (q & 63);
+        const int per_row = C->nmem * C->nedge;  // wavefronts of one row's face runs
+        r = w / per_row;
+        mm = (w - r * per_row) / C->nedge;
+        x = 64 * C->edge[w - r * per_row - mm * C->nedge] + (q & 63);
     }
     const int m = __builtin_amdgcn_readfirstlane(mm);
     const bool lane = m < C->nmem && r < R && x < X;

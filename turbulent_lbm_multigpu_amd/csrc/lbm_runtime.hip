@@ -1368,12 +1368,12 @@ static int group_upload_descs(lbm_group *g) {
             while (R > 1 && M * lanes_of(R) > g->wg) --R;
             xc.rows = R;
             xc.lanes = lanes_of(R);
-        } else {  // one row in 64-cell runs: the face runs first, then the interior ones
-            xc.rows = 1;
-            xc.nedge = chain_edge_runs(X, xc.edge);
-            xc.ninner = std::max(0, (X - 2) / 64 - 1);
+        } else {  // rows in 64-cell runs: the face runs of as many rows as fill the work-group, then the interior ones
             const int waves = g->wg / 64;
-            xc.ipasses = (M * xc.ninner + waves - 1) / waves;
+            xc.nedge = chain_edge_runs(X, xc.edge);
+            xc.rows = std::max(1, std::min(waves / (M * xc.nedge), kChainRows / M));
+            xc.ninner = std::max(0, (X - 2) / 64 - 1);
+            xc.ipasses = (M * xc.ninner * xc.rows + waves - 1) / waves;
         }
         const int R = xc.rows;
         xc.tile_begin = (int)tiles.size();

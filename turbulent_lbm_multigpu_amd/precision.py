@@ -28,9 +28,10 @@ STORAGES = {"f64": _capi.STORAGE_F64, "f32": _capi.STORAGE_F32, "f16": _capi.STO
 
 
 def build_channel(dims, storage, *, tau=0.51, cs=0.17, gx=1e-6, sphere_seed=20260101, fill=0.05, rmin=3.0,
-                  rmax=8.0, amp=0.0, seed=0, kernel_variant=0, work_group_size=256):
+                  rmax=8.0, amp=0.0, seed=0, kernel_variant=0, work_group_size=256, fused_x=True):
     """One periodic channel in a group of one subdomain whose two x comms point
-    at itself (the periodic seam is a ghost-layer exchange)."""
+    at itself (the periodic seam is a ghost-layer exchange, done inside the
+    step kernel with fused_x)."""
     sub = dec.decompose(dims, (1, 1, 1), periodic=(True, False, False))[0]
     bc = [sub.bc[0][0], sub.bc[0][1], sub.bc[1][0], sub.bc[1][1], sub.bc[2][0], sub.bc[2][1]]
     s = LbmSolver(dims, bc, inv_tau=1.0 / tau, tau=tau, smagorinsky_cs=cs, gravitation=(gx, 0.0, 0.0),
@@ -45,6 +46,7 @@ def build_channel(dims, storage, *, tau=0.51, cs=0.17, gx=1e-6, sphere_seed=2026
     g.set_exchange_mode(0)
     g.sync(after_beta=False)   # ghost layers start as copies of their periodic images
     g.set_exchange_mode(1)
+    g.set_fused_x(fused_x)
     return s, g, achieved
 
 
@@ -95,8 +97,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--storages", nargs="+", default=["f64", "f32", "f16", "bf16"])
     ap.add_argument("--variant", type=int, default=3, help="kernel variant (3: non-temporal loads and stores)")
+    ap.add_argument("--fused-x", type=int, default=1, choices=[0, 1],
+                    help="periodic x faces inside the step kernel (1) or by a copy launch (0)")
     a = ap.parse_args()
-    res = run(tuple(a.dims), a.steps, tuple(a.storages), kernel_variant=a.variant)
+    res = run(tuple(a.dims), a.steps, tuple(a.storages), kernel_variant=a.variant, fused_x=bool(a.fused_x))
     print(json.dumps({"workload": f"LES channel {a.dims[0]}x{a.dims[1]}x{a.dims[2]}, periodic x, g_x 1e-6, "
                                   f"tau 0.51, Cs 0.17, 5% seeded spheres r 3-8, {a.steps} steps",
                       "results": res}))
