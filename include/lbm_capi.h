@@ -189,14 +189,15 @@ int lbm_group_add_comm(lbm_group *g, int member, int dst_member, const int send_
 /* exchange mode: 0 = all 19 populations after alpha (reference),
  * 1 = only the 5 populations an inner cell reads from a ghost */
 int lbm_group_set_exchange_mode(lbm_group *g, int mode);
-/* x faces exchanged inside the step kernel (new): where a comm joins two
- * equally sized members across x with the reference's face geometry, the
- * sender writes the receiver's ghost layer during alpha and its inner layer
- * during beta, and the x-face copy launch disappears (the x faces are the
- * strided ones: one element per cache line).  Every slot a FLUID cell reads
- * is identical to the copy-based exchange; ghost cells of a fed face no
- * longer pass populations with an x component through to the wrapped
- * boundary slots (never read by FLUID cells).  lbm_group_sync still copies. */
+/* x faces exchanged inside the step kernel (new): equally sized members
+ * joined across x with the reference's face geometry form chains (at most 8,
+ * lines or periodic rings); one work-group holds the same rows of every
+ * member of a chain and hands the face values over in LDS, so the x-face copy
+ * launch disappears (the x faces are the strided ones: one element per cache
+ * line).  Every slot of every member is identical to the copy-based
+ * exchange.  Chains whose rows (or, for wide rows, whose 64-cell face runs)
+ * do not fit the work-group keep their copies.  lbm_group_sync still
+ * copies. */
 int lbm_group_set_fused_x(lbm_group *g, int enable);
 /* replay the step pair as a hipGraph (1) or launch eagerly (0) */
 int lbm_group_set_graph(lbm_group *g, int enable);

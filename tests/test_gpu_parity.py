@@ -373,6 +373,28 @@ def test_fused_x_with_frozen_member(domain_size, nums):
     fused_vs_copy(domain_size, nums, 1, stops=stops, step_runs=(5, 6))
 
 
+def test_fused_x_toggled_between_graph_replays():
+    """x faces switched into the kernel and back out between steps of a group
+    that replays a captured graph (the graph is captured again whenever a
+    table it holds moves): all slots equal to a group that always copies."""
+    say("case: fused x toggled between graph replays")
+    kw = dict(inv_tau=1 / 0.52, tau=0.52, smagorinsky_cs=0.17)
+    ref, _, _ = build_gpu_group((32, 28, 24), (2, 2, 2), lid=0.08, amp=1e-3, seed=11, mode=1, **kw)
+    tog, _, _ = build_gpu_group((32, 28, 24), (2, 2, 2), lid=0.08, amp=1e-3, seed=11, mode=1, **kw)
+    for steps, fused in ((4, False), (5, True), (3, False), (6, True)):
+        tog.set_fused_x(fused)
+        ref.step(steps)
+        tog.step(steps)
+        ref.wait()
+        tog.wait()
+        for a, b in zip(ref.solvers, tog.solvers):
+            np.testing.assert_array_equal(gpu_dd(a), gpu_dd(b))
+    for g in (ref, tog):
+        g.close()
+        for s in g.solvers:
+            s.close()
+
+
 def test_group_2x2x2_fused_matches_oracle_decomposition():
     """The fused x faces against the oracle's copy-based decomposition (all
     slots, exchange of all 19 populations)."""

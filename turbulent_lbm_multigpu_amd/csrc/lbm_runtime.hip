@@ -418,7 +418,7 @@ struct lbm_group {
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
-    std::vector<int> graph_shape;  // launch sizes baked into the captured graph
+    std::vector<long long> graph_shape;  // launch sizes and table addresses baked into the captured graph
     bool timing = false;
     struct Ev {
         hipEvent_t a, b;
@@ -1723,7 +1723,6 @@ static void plan_chains(lbm_group *g, const std::vector<char> &fuse) {
 static int build_plan(lbm_group *g) {
     std::vector<CopyOp> ops[2][4];
     std::vector<HOp> hops[2][4];
-    group_drop_graph(g);  // the graph holds the old tables
     std::vector<const GroupComm *> partner(g->comms.size(), nullptr);
     std::vector<char> fuse(g->comms.size(), 0);
     for (size_t ci = 0; ci < g->comms.size(); ++ci) {
@@ -1834,12 +1833,18 @@ static int group_prepare(lbm_group *g) {
     return LBM_OK;
 }
 
-static std::vector<int> group_shape(const lbm_group *g) {
-    std::vector<int> v = {g->ntiles, g->wg, g->variant, g->lds};
+// Everything a captured graph bakes into its launches: sizes and the device
+// tables' addresses (a table that grew was reallocated; its contents are read
+// at replay time and may change in place).
+static std::vector<long long> group_shape(const lbm_group *g) {
+    std::vector<long long> v = {g->ntiles, g->wg, g->variant, g->lds, (long long)(uintptr_t)g->d_descs,
+                                (long long)(uintptr_t)g->d_tiles, (long long)(uintptr_t)g->d_chains,
+                                (long long)(uintptr_t)g->d_ctr};
     for (int k = 0; k < 2; ++k)
         for (int p = 0; p < 4; ++p) {
             v.push_back(g->nops[k][p]);
             v.push_back(g->nblocks[k][p]);
+            v.push_back((long long)(uintptr_t)g->d_ops[k][p]);
         }
     return v;
 }
