@@ -8,7 +8,10 @@
 // the previous one left (src/CController.hpp:265-383) — is played out and
 // compared with the merged plan, whose copies all read the state before the
 // exchange.  Prints one line per case: "<case> merged|phased ok" or a
-// mismatch; exit status 1 on any mismatch.
+// mismatch; exit status 1 on any mismatch.  The chain planning (joins, lines
+// and rings, tile shapes) is checked against known cases first, one
+// "chains <case> ok|FAIL" line each.
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -126,8 +129,87 @@ static int run(const Case &c, bool &merged) {
     return 0;
 }
 
+// ---- chains of x-joined members ----------------------------------------------
+static int check(bool ok, const char *what) {
+    std::printf("chains %s %s\n", what, ok ? "ok" : "FAIL");
+    return ok ? 0 : 1;
+}
+
+// joins from pairs (a receives on its side s from b): both directions listed
+static std::vector<lbm::JoinCandidate> cands(std::initializer_list<std::array<int, 3>> l) {
+    std::vector<lbm::JoinCandidate> v;
+    for (const auto &t : l) v.push_back({t[0], t[1], t[2], 0x1fu});
+    return v;
+}
+
+static int run_chains() {
+    int f = 0;
+    using V = std::vector<std::vector<int>>;
+    {  // a line 0 - 1 - 2 along x: 1's x- from 0, 0's x+ from 1, ...
+        auto j = lbm::resolve_joins(3, cands({{1, 0, 0}, {0, 1, 1}, {2, 0, 1}, {1, 1, 2}}));
+        auto ch = lbm::find_chains(j, {128, 128, 128}, 512);
+        f += check(ch == V{{0, 1, 2}} && j[0].nb[0] == -1 && j[0].nb[1] == 1 && j[2].nb[1] == -1, "line of three");
+    }
+    {  // a ring of two (x periodic across two members): each feeds both faces of the other
+        auto j = lbm::resolve_joins(2, cands({{0, 0, 1}, {0, 1, 1}, {1, 0, 0}, {1, 1, 0}}));
+        auto ch = lbm::find_chains(j, {100, 100}, 256);
+        f += check(ch == V{{0, 1}} && j[0].nb[0] == 1 && j[0].nb[1] == 1, "ring of two");
+    }
+    {  // one member periodic in x: a ring of one
+        auto j = lbm::resolve_joins(1, cands({{0, 0, 0}, {0, 1, 0}}));
+        auto ch = lbm::find_chains(j, {24}, 256);
+        f += check(ch == V{{0}} && j[0].nb[0] == 0 && j[0].nb[1] == 0, "ring of one");
+    }
+    {  // two comms on member 1's x- face: that face joins nothing, nor do the partners' faces
+        auto j = lbm::resolve_joins(3, cands({{1, 0, 0}, {1, 0, 2}, {0, 1, 1}, {2, 1, 1}}));
+        auto ch = lbm::find_chains(j, {64, 64, 64}, 256);
+        f += check(ch.empty() && j[1].nb[0] == -1 && j[0].nb[1] == -1 && j[2].nb[1] == -1, "conflicting face");
+    }
+    {  // a join one side does not name back is dropped
+        auto j = lbm::resolve_joins(3, cands({{1, 0, 0}, {0, 1, 2}}));
+        auto ch = lbm::find_chains(j, {64, 64, 64}, 256);
+        f += check(ch.empty() && j[1].nb[0] == -1 && j[1].mask[0] == 0u && j[0].nb[1] == -1, "one-sided join");
+    }
+    {  // nine members in a line: longer than kMaxChain, no chain
+        std::vector<std::array<int, 3>> l;
+        std::vector<lbm::JoinCandidate> c;
+        for (int i = 1; i < 9; ++i) {
+            c.push_back({i, 0, i - 1, 1u});
+            c.push_back({i - 1, 1, i, 1u});
+        }
+        auto j = lbm::resolve_joins(9, c);
+        auto ch = lbm::find_chains(j, std::vector<int>(9, 16), 256);
+        bool cleared = true;
+        for (const auto &x : j) cleared = cleared && x.nb[0] == -1 && x.nb[1] == -1;
+        f += check(ch.empty() && cleared, "nine in a line");
+    }
+    {  // tile shapes
+        lbm::ChainShape a = lbm::chain_shape(2, 128, 256);  // the benchmark: one row of both, 128 lanes each
+        f += check(a.fits && a.rows == 1 && a.lanes == 128 && a.nedge == 0, "shape 2 x 128, wg 256");
+        lbm::ChainShape b = lbm::chain_shape(2, 14, 256);   // 8 rows of 14 per member -> 112 -> 128 lanes
+        f += check(b.fits && b.rows == 8 && b.lanes == 128, "shape 2 x 14, wg 256");
+        lbm::ChainShape c = lbm::chain_shape(2, 128, 128);  // rows too wide, face runs {0, 1} x 2 > 2 waves
+        f += check(!c.fits, "shape 2 x 128, wg 128");
+        lbm::ChainShape d = lbm::chain_shape(1, 384, 256);  // runs {0, 5}, 2 rows per handoff tile, runs 1..4
+        f += check(d.fits && d.nedge == 2 && d.edge[0] == 0 && d.edge[1] == 5 && d.rows == 2 && d.ninner == 4 &&
+                       d.ipasses == 2,
+                   "shape 1 x 384, wg 256");
+        lbm::ChainShape e = lbm::chain_shape(2, 130, 256);  // runs {0, 2}: one row per handoff tile, run 1 inside
+        f += check(e.fits && e.nedge == 2 && e.rows == 1 && e.ninner == 1 && e.ipasses == 1, "shape 2 x 130, wg 256");
+        lbm::ChainShape g = lbm::chain_shape(1, 129, 256);  // one row on 192 lanes
+        f += check(g.fits && g.nedge == 0 && g.rows == 1 && g.lanes == 192, "shape 1 x 129, wg 256");
+        lbm::ChainShape k = lbm::chain_shape(1, 321, 256);  // X - 2 = 319 and X - 1 = 320 in runs 4 and 5
+        f += check(k.fits && k.nedge == 3 && k.edge[0] == 0 && k.edge[1] == 4 && k.edge[2] == 5 && k.rows == 1 &&
+                       k.ninner == 3 && k.ipasses == 1,
+                   "shape 1 x 321, wg 256");
+        lbm::ChainShape h = lbm::chain_shape(9, 16, 1024);
+        f += check(!h.fits, "shape 9 members");
+    }
+    return f;
+}
+
 int main() {
-    int fails = 0;
+    int fails = run_chains();
     const int nums_list[][3] = {{2, 2, 2}, {3, 1, 2}, {1, 3, 3}, {2, 2, 1}, {3, 3, 3}, {4, 1, 1}, {1, 1, 1}};
     const bool per_list[][3] = {{false, false, false}, {true, false, false}, {true, true, true}, {false, true, false}};
     for (const auto &nums : nums_list)

@@ -1,4 +1,5 @@
-"""The exchange planner on the CPU (g++, no GPU): the reference's sequential
+"""The exchange planner on the CPU (g++, no GPU): the chains of x-joined
+members the step kernel exchanges in LDS, and the reference's sequential
 x, y, z ghost-layer copy phases (src/CController.hpp:265-383) against the
 merged plan that runs them as one launch, on every slot of block
 decompositions built like CManager::initSimulation (src/CManager.hpp:78-204),
@@ -30,9 +31,24 @@ def selftest(tmp_path_factory):
     return exe
 
 
+def run_selftest(exe):
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    return out.returncode, out.stdout.strip().splitlines()
+
+
 def test_merged_plan_equals_sequential_phases(selftest):
-    out = subprocess.run([selftest], capture_output=True, text=True, timeout=120)
-    lines = out.stdout.strip().splitlines()
-    assert out.returncode == 0, "\n".join(ln for ln in lines if "MISMATCH" in ln)
+    rc, lines = run_selftest(selftest)
+    lines = [ln for ln in lines if not ln.startswith("chains ")]
+    assert rc == 0, "\n".join(ln for ln in lines if "MISMATCH" in ln)
     assert len(lines) == 7 * 4 * 2 * 2 * 2
     assert all(ln.endswith("merged ok") for ln in lines), [ln for ln in lines if not ln.endswith("merged ok")]
+
+
+def test_chain_planning(selftest):
+    """Joins (conflicting and one-sided ones dropped), lines and rings in x
+    order, chains longer than 8 left out, and the chain-tile shapes (whole
+    rows, or face runs plus interior tiles for wide rows)."""
+    rc, lines = run_selftest(selftest)
+    chains = [ln for ln in lines if ln.startswith("chains ")]
+    assert len(chains) == 14
+    assert all(ln.endswith(" ok") for ln in chains), [ln for ln in chains if not ln.endswith(" ok")]

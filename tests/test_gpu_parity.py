@@ -416,7 +416,7 @@ def test_group_2x2x2_fused_matches_oracle_decomposition():
         s.close()
 
 
-@pytest.mark.parametrize("dims", [(24, 18, 16), (300, 9, 7)])
+@pytest.mark.parametrize("dims", [(24, 18, 16), (300, 9, 7), (321, 7, 6)])
 def test_fused_x_periodic_channel(dims):
     """A periodic channel (self comms across x) with fused x faces, a ring of
     one (whole rows, and two-row run tiles with interior tiles when X = 300, the last row tile half full):

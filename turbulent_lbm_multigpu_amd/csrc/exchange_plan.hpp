@@ -1,7 +1,8 @@
-// Host-side exchange plan: the ghost-layer copies of one half-step as boxes,
-// and the merge of the reference's sequential x, y, z copy phases into one
-// set of independent copies.  Plain C++ (no HIP), so the planner is also
-// tested on the CPU.
+// Host-side exchange plan: the chains of x-joined members the step kernel
+// exchanges in LDS, the ghost-layer copies of one half-step as boxes, and the
+// merge of the reference's sequential x, y, z copy phases into one set of
+// independent copies.  Plain C++ (no HIP), so the planner is also tested on
+// the CPU.
 #pragma once
 
 #include <algorithm>
@@ -10,6 +11,130 @@
 #include <vector>
 
 namespace lbm {
+
+// ---- chains of x-joined members ---------------------------------------------
+// Same-sized members joined across x faces form chains (lines, or rings where
+// x is periodic); the step kernel's chain tiles hand the face values over in
+// LDS (see chain_tile in lbm_device.hpp).
+constexpr int kMaxChain = 8;    // members of one chain
+constexpr int kChainRows = 16;  // members x rows of one chain tile (LDS capacity)
+
+// Joins of one member: nb[side] = the member whose inner column feeds my x-
+// (side 0) / x+ (side 1) ghost layer inside the step kernel, or -1; mask[side]
+// = the slots that ghost layer takes after alpha.
+struct XJoin {
+    int nb[2];
+    uint32_t mask[2];
+};
+
+// A fusable x comm: `member` receives across its face `side` from `dst`.
+struct JoinCandidate {
+    int member, side, dst;
+    uint32_t mask;
+};
+
+// The joins the candidates make: a face named by two candidates joins
+// nothing, and a join holds only when both members name each other.
+inline std::vector<XJoin> resolve_joins(int nm, const std::vector<JoinCandidate> &cand) {
+    std::vector<XJoin> j((size_t)nm, XJoin{{-1, -1}, {0u, 0u}});
+    for (const JoinCandidate &c : cand) {
+        XJoin &f = j[c.member];
+        if (f.nb[c.side] != -1) {
+            f.nb[c.side] = -2;  // conflict
+            continue;
+        }
+        f.nb[c.side] = c.dst;
+        f.mask[c.side] = c.mask;
+    }
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < nm; ++i)
+            for (int side = 0; side < 2; ++side) {
+                const int k = j[i].nb[side];
+                if (k == -2 || (k >= 0 && j[k].nb[1 - side] != i)) j[i].nb[side] = -1;
+            }
+    for (XJoin &f : j)
+        for (int side = 0; side < 2; ++side)
+            if (f.nb[side] < 0) f.mask[side] = 0u;
+    return j;
+}
+
+// 64-cell runs of an x row that hold columns 0, 1, X-2 and X-1 (distinct).
+inline int chain_edge_runs(int X, int *edge) {
+    int n = 0;
+    const int cand[3] = {0, (X - 2) / 64, (X - 1) / 64};
+    for (int c : cand) {
+        bool seen = false;
+        for (int k = 0; k < n; ++k) seen |= edge[k] == c;
+        if (!seen) edge[n++] = c;
+    }
+    return n;
+}
+
+// How a chain of M members with rows of X cells is tiled for work-groups of
+// wg work-items:
+//   whole rows: `rows` rows of every member, `lanes` work-items per member
+//     (rows * X rounded up to whole wavefronts), all in one work-group;
+//   run tiles (rows too wide): the 64-cell runs holding columns 0, 1, X-2,
+//     X-1 of `rows` rows of every member per handoff tile, their interior
+//     runs 1 .. ninner in `ipasses` tiles of their own.
+struct ChainShape {
+    bool fits;
+    int rows, lanes, nedge, edge[3], ninner, ipasses;
+};
+
+inline ChainShape chain_shape(int M, int X, int wg) {
+    ChainShape c{false, 0, 0, 0, {0, 0, 0}, 0, 0};
+    if (M < 1 || M > kMaxChain || X < 4) return c;
+    const int waves = wg / 64;
+    auto lanes_of = [X](int r) { return (r * X + 63) / 64 * 64; };
+    if (M * lanes_of(1) <= wg) {
+        int R = kChainRows / M;
+        while (R > 1 && M * lanes_of(R) > wg) --R;
+        c.fits = true;
+        c.rows = R;
+        c.lanes = lanes_of(R);
+        return c;
+    }
+    c.nedge = chain_edge_runs(X, c.edge);
+    if (M * c.nedge > waves) {
+        c.nedge = 0;
+        return c;
+    }
+    c.fits = true;
+    c.rows = std::max(1, std::min(waves / (M * c.nedge), kChainRows / M));
+    c.ninner = std::max(0, (X - 2) / 64 - 1);
+    c.ipasses = (M * c.ninner * c.rows + waves - 1) / waves;
+    return c;
+}
+
+// The chains the joins form, each in x order: lines from the member with no
+// left partner, then rings.  A chain that does not fit (chain_shape with its
+// members' X) is left out and its members' joins are cleared.
+inline std::vector<std::vector<int>> find_chains(std::vector<XJoin> &j, const std::vector<int> &X, int wg) {
+    const int nm = (int)j.size();
+    std::vector<char> seen((size_t)nm, 0);
+    auto walk = [&](int start) {
+        std::vector<int> ch;
+        for (int cur = start; cur >= 0 && !seen[cur]; cur = j[cur].nb[1]) {
+            seen[cur] = 1;
+            ch.push_back(cur);
+        }
+        return ch;
+    };
+    std::vector<std::vector<int>> found, kept;
+    for (int i = 0; i < nm; ++i)
+        if (!seen[i] && j[i].nb[0] < 0 && j[i].nb[1] >= 0) found.push_back(walk(i));
+    for (int i = 0; i < nm; ++i)
+        if (!seen[i] && j[i].nb[0] >= 0) found.push_back(walk(i));
+    for (auto &ch : found) {
+        if (chain_shape((int)ch.size(), X[ch[0]], wg).fits) {
+            kept.push_back(ch);
+            continue;
+        }
+        for (int m : ch) j[m] = XJoin{{-1, -1}, {0u, 0u}};
+    }
+    return kept;
+}
 
 // One copy of the exchange plan on the host: the box `sz` at `so` of member
 // sm to `dof` of member dm, populations in `mask`.

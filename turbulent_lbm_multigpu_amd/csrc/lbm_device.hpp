@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "exchange_plan.hpp"  // kMaxChain, kChainRows
+
 namespace lbm {
 
 constexpr int kFlagObstacle = 1;
@@ -251,9 +253,8 @@ struct DomDev {
 // member, so both columns next to each shared face are in one work-group: the
 // face values move through LDS and each population slot is stored only by the
 // work-item that owns it in the AA pattern (whole cache lines, no stores into
-// another subdomain), with the result of the copy-based exchange.
-constexpr int kMaxChain = 8;    // members of one chain
-constexpr int kChainRows = 16;  // members x rows of one chain tile (LDS capacity)
+// another subdomain), with the result of the copy-based exchange.  The host
+// side (joins, chains, tile shape) is in exchange_plan.hpp.
 struct XChain {
     int nmem, rows, nx, tile_begin;
     int lanes;             // whole-row tiles: work-items per member, rows * nx rounded up to whole wavefronts

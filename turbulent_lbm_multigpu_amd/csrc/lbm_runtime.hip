@@ -406,11 +406,7 @@ struct lbm_group {
     size_t ops_cap[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     int nops[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     int nblocks[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-    struct XFuse {
-        int nb[2];          // member whose column feeds my x- / x+ ghost layer in the step kernel, or -1
-        unsigned mask[2];   // slots that ghost layer takes after alpha
-    };
-    std::vector<XFuse> xfuse;                 // per member, from the plan
+    std::vector<XJoin> xfuse;                 // per member, from the plan
     std::vector<std::vector<int>> chains;     // joined members in x order, from the plan
     XChain *d_chains = nullptr;
     size_t chains_cap = 0;
@@ -1306,18 +1302,6 @@ static hipError_t grow_table(T *&ptr, size_t &cap, size_t n) {
     return e;
 }
 
-// 64-cell runs of an x row that hold columns 0, 1, X-2 and X-1 (distinct).
-static int chain_edge_runs(int X, int *edge) {
-    int n = 0;
-    const int cand[3] = {0, (X - 2) / 64, (X - 1) / 64};
-    for (int c : cand) {
-        bool seen = false;
-        for (int k = 0; k < n; ++k) seen |= edge[k] == c;
-        if (!seen) edge[n++] = c;
-    }
-    return n;
-}
-
 // Descriptor and tile tables of the fused launch for the group's current
 // members and launch shape (work-group size and cells per lane); also
 // re-publishes the host half-step count to the device counter.
@@ -1359,22 +1343,16 @@ static int group_upload_descs(lbm_group *g) {
         const lbm_solver *m0 = g->members[ch[0]];
         const int X = m0->cfg.dims[0];
         const long long rows = (long long)m0->cfg.dims[1] * m0->cfg.dims[2];
-        auto lanes_of = [X](int r) { return (r * X + 63) / 64 * 64; };
+        const ChainShape sh = chain_shape(M, X, g->wg);  // fits: the plan kept only chains that do
         XChain xc{};
         xc.nmem = M;
         xc.nx = X;
-        if (M * lanes_of(1) <= g->wg) {  // whole rows
-            int R = kChainRows / M;
-            while (R > 1 && M * lanes_of(R) > g->wg) --R;
-            xc.rows = R;
-            xc.lanes = lanes_of(R);
-        } else {  // rows in 64-cell runs: the face runs of as many rows as fill the work-group, then the interior ones
-            const int waves = g->wg / 64;
-            xc.nedge = chain_edge_runs(X, xc.edge);
-            xc.rows = std::max(1, std::min(waves / (M * xc.nedge), kChainRows / M));
-            xc.ninner = std::max(0, (X - 2) / 64 - 1);
-            xc.ipasses = (M * xc.ninner * xc.rows + waves - 1) / waves;
-        }
+        xc.rows = sh.rows;
+        xc.lanes = sh.lanes;
+        xc.nedge = sh.nedge;
+        for (int k = 0; k < 3; ++k) xc.edge[k] = sh.edge[k];
+        xc.ninner = sh.ninner;
+        xc.ipasses = sh.ipasses;
         const int R = xc.rows;
         xc.tile_begin = (int)tiles.size();
         for (int k = 0; k < M; ++k) {
@@ -1662,62 +1640,18 @@ static uint32_t alpha_mask(const lbm_group *g, const GroupComm &c) {
 }
 
 // Joins the fusable x faces into chains (lines, or rings where x is
-// periodic) and keeps those a chain tile can hold: at most kMaxChain
-// members, and either whole rows of all of them within a work-group or, for
-// wider rows, the 64-cell runs next to the faces of all of them.
+// periodic) and keeps those a chain tile can hold (chain_shape).
 static void plan_chains(lbm_group *g, const std::vector<char> &fuse) {
-    const int nm = (int)g->members.size();
-    g->xfuse.assign((size_t)nm, lbm_group::XFuse{{-1, -1}, {0u, 0u}});
-    g->chains.clear();
+    std::vector<JoinCandidate> cand;
     for (size_t ci = 0; ci < g->comms.size(); ++ci) {
         if (!fuse[ci]) continue;
         const GroupComm &c = g->comms[ci];
-        const int side = c.normal[0] == 1 ? 0 : 1;  // inward normal +1: my x- face
-        lbm_group::XFuse &f = g->xfuse[c.member];
-        if (f.nb[side] >= 0) {  // two comms on one face: neither is joined
-            f.nb[side] = -2;
-            continue;
-        }
-        f.nb[side] = c.dst;
-        f.mask[side] = alpha_mask(g, c);
+        cand.push_back({c.member, c.normal[0] == 1 ? 0 : 1, c.dst, alpha_mask(g, c)});  // inward +1: my x- face
     }
-    // a join holds only when both sides agree
-    for (int i = 0; i < nm; ++i)
-        for (int side = 0; side < 2; ++side) {
-            const int j = g->xfuse[i].nb[side];
-            if (j == -2 || (j >= 0 && g->xfuse[j].nb[1 - side] != i)) g->xfuse[i].nb[side] = -1;
-        }
-    for (int i = 0; i < nm; ++i)
-        for (int side = 0; side < 2; ++side) {
-            const int j = g->xfuse[i].nb[side];
-            if (j >= 0 && g->xfuse[j].nb[1 - side] != i) g->xfuse[i].nb[side] = -1;
-        }
-    std::vector<char> seen((size_t)nm, 0);
-    auto walk = [&](int start) {
-        std::vector<int> ch;
-        for (int cur = start; cur >= 0 && !seen[cur]; cur = g->xfuse[cur].nb[1]) {
-            seen[cur] = 1;
-            ch.push_back(cur);
-        }
-        return ch;
-    };
-    std::vector<std::vector<int>> found;
-    for (int i = 0; i < nm; ++i)  // lines start at a member with no left partner
-        if (!seen[i] && g->xfuse[i].nb[0] < 0 && g->xfuse[i].nb[1] >= 0) found.push_back(walk(i));
-    for (int i = 0; i < nm; ++i)  // what is left are rings
-        if (!seen[i] && g->xfuse[i].nb[0] >= 0) found.push_back(walk(i));
-    for (auto &ch : found) {
-        const int M = (int)ch.size();
-        const int X = g->members[ch[0]]->cfg.dims[0];
-        int edge[3];
-        const bool rows_fit = M * ((X + 63) / 64 * 64) <= g->wg;
-        const bool runs_fit = M * chain_edge_runs(X, edge) <= g->wg / 64;
-        if (M <= kMaxChain && (rows_fit || runs_fit)) {
-            g->chains.push_back(ch);
-            continue;
-        }
-        for (int mem : ch) g->xfuse[mem] = lbm_group::XFuse{{-1, -1}, {0u, 0u}};
-    }
+    g->xfuse = resolve_joins((int)g->members.size(), cand);
+    std::vector<int> X;
+    for (const lbm_solver *m : g->members) X.push_back(m->cfg.dims[0]);
+    g->chains = find_chains(g->xfuse, X, g->wg);
 }
 
 static int build_plan(lbm_group *g) {
