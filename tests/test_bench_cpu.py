@@ -1,0 +1,34 @@
+"""bench.py's contract on the CPU: one GPU only, and the committed HBM
+traffic summary is the one measured with the benchmark's default launch
+configuration (so the default run reports `roofline.traffic`)."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120, cwd=ROOT)
+    assert out.returncode == 2
+    assert "one GPU" in out.stderr
+    assert out.stdout == ""
+
+
+def test_traffic_summary_matches_the_default_configuration():
+    # the defaults of bench.py's argument parser
+    traffic = bench.measured_traffic(exchange_mode=1, kernel_variant=3, wg=256, fused_x=True)
+    assert traffic is not None
+    algorithmic = 152 * 256 ** 3
+    assert algorithmic <= traffic < 1.05 * algorithmic
+
+
+def test_traffic_summary_is_not_used_for_another_configuration():
+    assert bench.measured_traffic(exchange_mode=1, kernel_variant=0, wg=256, fused_x=True) is None
+    assert bench.measured_traffic(exchange_mode=0, kernel_variant=3, wg=256, fused_x=True) is None
+    assert bench.measured_traffic(exchange_mode=1, kernel_variant=3, wg=128, fused_x=True) is None
+    assert bench.measured_traffic(exchange_mode=1, kernel_variant=3, wg=256, fused_x=False) is None
