@@ -26,12 +26,11 @@ def run(*args, cwd=None, timeout=120):
 
 @pytest.mark.parametrize("fused_x", [0, 1])
 def test_validation_mode_all_subdomains_match(tmp_path, fused_x):
-    conf = os.path.join(DATA, "conf_validate_les.xml")
-    if fused_x:  # x faces exchanged inside the step kernel: interiors stay identical
-        text = open(conf).read().replace("<exchange-mode>1</exchange-mode>",
-                                         "<exchange-mode>1</exchange-mode><fused-x>1</fused-x>")
-        conf = str(tmp_path / "conf_fused.xml")
-        open(conf, "w").write(text)
+    # x faces inside the step kernel (the default) or by the copy launch: the same cells
+    text = open(os.path.join(DATA, "conf_validate_les.xml")).read().replace(
+        "<exchange-mode>1</exchange-mode>", f"<exchange-mode>1</exchange-mode><fused-x>{fused_x}</fused-x>")
+    conf = str(tmp_path / "conf_fused.xml")
+    open(conf, "w").write(text)
     r = run("-c", conf, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = re.findall(r"NUMBER OF FAILED CELLS/TOTAL NUMBER OF CELLS: (\d+)/(\d+)", r.stdout)

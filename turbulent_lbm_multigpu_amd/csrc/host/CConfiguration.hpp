@@ -143,7 +143,7 @@ public:
     double perturbation_amplitude = 0.0;
     unsigned long long perturbation_seed = 0;
     int exchange_mode = 0;  // 0: all 19 populations after alpha (reference), 1: the 5 an inner cell reads
-    int fused_x = 0;        // 1: x faces exchanged inside the step kernel
+    int fused_x = 1;        // 1: x faces exchanged inside the step kernel (same results as 0, the copy launch)
 
     CConfiguration() {}
     explicit CConfiguration(const std::string &file_name) { loadFile(file_name); }
