@@ -267,6 +267,40 @@ def test_group_decomposed_equals_monolithic_and_minimal_exchange():
     one.close()
 
 
+def test_full_size_benchmark_decomposition_equals_monolithic():
+    """The benchmark workload at its full size (256^3 LES as 2x2x2 subdomains
+    of 128^3, kernel variant 3, x faces in the kernel, exchange mode 1, one
+    copy launch per half-step, graph replay) against a single 254^3 domain:
+    the reference's validation invariant (src/main.cpp:358-405), interior rho
+    and j bit-identical after 10 steps."""
+    say("case: full-size benchmark group vs monolithic 254^3, 10 steps")
+    import bench
+
+    w = bench.WORKLOAD
+    grp = bench.build_group(exchange_mode=1, graph=True, wg=256, kernel_variant=3, fused_x=True)
+    mono = dec.monolithic_size(w["domain"], w["nums"])
+    one = LbmSolver(mono, (1,) * 6, inv_tau=1.0 / w["tau"], tau=w["tau"], smagorinsky_cs=w["cs"],
+                    lid_velocity=w["lid"], kernel_variant=3)
+    one.init_perturbed(w["amp"], w["seed"])
+    o, sz = dec.lid_region(mono)
+    one.set(_capi.FIELD_FLAGS, np.full(sz[::-1], orc.FLAG_INJECTION, np.int32), o, sz)
+    grp.step(10)
+    one.run(10)
+    grp.wait()
+    one.wait()
+    for sd, s in zip(dec.decompose(w["domain"], w["nums"]), grp.solvers):
+        s3, ko = sd.domain.size, dec.key_origin(sd)
+        size = tuple(s3[a] - 2 for a in range(3))
+        rho_a, j_a = s.moments((1, 1, 1), size)
+        rho_b, j_b = one.moments(tuple(ko[a] + 1 for a in range(3)), size)
+        np.testing.assert_array_equal(rho_a, rho_b)
+        np.testing.assert_array_equal(j_a, j_b)
+    grp.close()
+    for s in grp.solvers:
+        s.close()
+    one.close()
+
+
 @pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
 def test_shifted_storage_error_vs_fp64(storage):
     """FP16/BF16 storage of f - w_i with FP32 collision stays close to FP64."""
