@@ -301,6 +301,27 @@ def test_full_size_benchmark_decomposition_equals_monolithic():
     one.close()
 
 
+@pytest.mark.parametrize("storage", [_capi.STORAGE_F32, _capi.STORAGE_F16_SHIFTED])
+def test_full_size_precision_channel_fused_equals_copies(storage):
+    """The precision workload at its full size (384x192x192 periodic LES
+    channel with spheres and a body force): x faces in the kernel by run
+    tiles (a 384-cell row is wider than a work-group) equal the copy
+    exchange, rho and j bit-identical after 6 steps."""
+    say(f"case: full-size channel 384x192x192 storage {storage}, fused vs copies")
+    from turbulent_lbm_multigpu_amd import precision
+
+    out = []
+    for fused in (True, False):
+        s, g, _ = precision.build_channel((384, 192, 192), storage, kernel_variant=3, fused_x=fused)
+        g.step(6)
+        g.wait()
+        out.append(s.moments())
+        g.close()
+        s.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
 def test_shifted_storage_error_vs_fp64(storage):
     """FP16/BF16 storage of f - w_i with FP32 collision stays close to FP64."""
