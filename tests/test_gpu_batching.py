@@ -125,3 +125,18 @@ def test_batched_cavities_equal_sequential():
     for (r1, j1), (r2, j2) in zip(fb, fs):
         np.testing.assert_array_equal(r1, r2)
         np.testing.assert_array_equal(j1, j2)
+
+
+def test_full_size_batch_equals_sequential():
+    """The batching workload at its full size (the benchmark's 512 seeded
+    cavities, edges 16..96): batched with the autotuner's shapes and one at a
+    time give bit-identical moments for every case after 60 steps."""
+    say("case: 512 cavities batched vs sequential, 60 steps")
+    cases = batch.make_cavities(512, 20260101)
+    rb, fb = batch.run_batched(cases, 60, autotune_pairs=1, keep=True)
+    rs, fs = batch.run_sequential(cases, 60, keep=True)
+    say(f"  batched {rb['launch']}, {rb['cells']} cells")
+    assert rb["autotune"] is not None
+    for (r1, j1), (r2, j2) in zip(fb, fs):
+        np.testing.assert_array_equal(r1, r2)
+        np.testing.assert_array_equal(j1, j2)
