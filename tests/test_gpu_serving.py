@@ -128,3 +128,22 @@ def test_failed_case_does_not_stop_the_others():
             assert np.isfinite(rho).all()
         st = srv.stats()
         assert st["failed"] == 1 and st["completed"] == 2 and st["bytes_in_use"] == 0
+
+
+def test_full_size_mix_alone_equals_in_the_mix():
+    """The serving workload at its full size (the benchmark's 24 seeded
+    requests: cavities and channels of 32^3..192^3, 500..5000 steps), FP16
+    storage: every case's rho and j are bit-identical served in the mix and
+    served alone."""
+    say("case: 24 full-size requests, FP16, in the mix vs alone")
+    reqs = serving.request_stream(24, 0.0, 20260101)
+    with serving.Server(200e9, chunk_ms=25.0) as srv:
+        mix = serving.serve(srv, reqs, storage="f16", keep_results=True)
+    assert mix.completed == len(reqs)
+    for k, r in enumerate(reqs):
+        with serving.Server(8e9, chunk_ms=25.0, max_active=1) as srv:
+            one = serving.serve(srv, [serving.Request(0.0, r.kind, r.edge, r.steps, "f16", r.seed, lid=r.lid,
+                                                      tau=r.tau)], keep_results=True)
+        np.testing.assert_array_equal(mix.results[k][0], one.results[0][0])
+        np.testing.assert_array_equal(mix.results[k][1], one.results[0][1])
+    say(f"  {len(reqs)} cases identical")
