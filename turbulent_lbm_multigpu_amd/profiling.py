@@ -25,7 +25,7 @@ import json
 import sys
 from collections import defaultdict
 from dataclasses import dataclass
-from typing import Dict, List, Sequence
+from typing import Dict, List, Optional, Sequence
 
 
 @dataclass
@@ -94,10 +94,13 @@ def counter_means(path: str) -> Dict[str, float]:
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def pmc_traffic(fetch_csv: str, write_csv: str, kernel: str, copy: str, copy_bytes: float) -> dict:
+def pmc_traffic(fetch_csv: str, write_csv: str, kernel: str, copy: str, copy_bytes: float,
+                fetch_scale: Optional[float] = None, write_scale: Optional[float] = None) -> dict:
     """HBM bytes per launch of every kernel whose name contains `kernel`.
     The copy kernel (`copy` in its name) reads and writes `copy_bytes` per
-    launch; its FETCH_SIZE and WRITE_SIZE readings give the scale factors."""
+    launch; its FETCH_SIZE and WRITE_SIZE readings give the scale factors,
+    unless they are given (a run without the copy kernel, scaled by another
+    run's calibration on the same part)."""
     fetch, write = counter_means(fetch_csv), counter_means(write_csv)
 
     def pick(table, key):
@@ -106,10 +109,11 @@ def pmc_traffic(fetch_csv: str, write_csv: str, kernel: str, copy: str, copy_byt
             raise KeyError(f"no kernel matching {key!r}")
         return hits
 
-    cf = list(pick(fetch, copy).values())[0]
-    cw = list(pick(write, copy).values())[0]
-    f_scale = copy_bytes / (cf * 1024.0)
-    w_scale = copy_bytes / (cw * 1024.0)
+    if fetch_scale is None or write_scale is None:
+        cf = list(pick(fetch, copy).values())[0]
+        cw = list(pick(write, copy).values())[0]
+    f_scale = fetch_scale if fetch_scale is not None else copy_bytes / (cf * 1024.0)
+    w_scale = write_scale if write_scale is not None else copy_bytes / (cw * 1024.0)
     out = {"fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
     for name, kib in pick(fetch, kernel).items():
         wk = write.get(name)
@@ -136,13 +140,16 @@ def main(argv=None):
     b.add_argument("--kernel", default="step_kernel")
     b.add_argument("--copy", default="stream_copy4_kernel<true>")
     b.add_argument("--copy-bytes", type=float, default=float(1 << 30))
+    b.add_argument("--fetch-scale", type=float, default=None, help="use this FETCH_SIZE scale (no copy kernel)")
+    b.add_argument("--write-scale", type=float, default=None, help="use this WRITE_SIZE scale (no copy kernel)")
     args = ap.parse_args(argv)
     if args.cmd == "ini":
         ev = events_from_kernel_trace(args.trace, args.filter)
         write_event_ini(ev, args.out)
         print(f"{len(ev)} events, {len(overlapping(ev))} overlapping pairs", file=sys.stderr)
     else:
-        print(json.dumps(pmc_traffic(args.fetch, args.write, args.kernel, args.copy, args.copy_bytes), indent=1))
+        print(json.dumps(pmc_traffic(args.fetch, args.write, args.kernel, args.copy, args.copy_bytes,
+                                     args.fetch_scale, args.write_scale), indent=1))
 
 
 if __name__ == "__main__":
