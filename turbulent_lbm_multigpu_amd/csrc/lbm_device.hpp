@@ -533,15 +533,52 @@ __device__ __forceinline__ void store_macros(const DomDev *D, unsigned g, int ce
 // One cell, element accesses.  With WRAP the neighbour index goes through the
 // linear wrap W (src/cl_programs/wrap.h:112-127); without it the block is far
 // enough from both ends of the domain for W to be the identity.
+// A wavefront whose 64 lanes hold the consecutive cells g0 .. g0+63, none
+// wrapping: each shifted beta access is then one run s .. s+63, and where the
+// run lies in one layout block its address is a wavefront-uniform base plus
+// the lane's byte offset, with no per-lane index arithmetic.  Used for 4- and
+// 8-byte storages (kRunAddressing): it cuts beta's vector instructions by a
+// third; the two-byte kernels, which need every wave of occupancy, lose more
+// to its registers than they gain.
+struct Run {
+    bool ld, st;  // every gather (g0 - d_i) / scatter (g0 + d_i) run lies in one layout block
+    unsigned g0;  // first cell of the wavefront (uniform)
+    unsigned lo;  // this lane's byte offset from it (lane * element bytes)
+};
+
+__device__ __forceinline__ bool run_in_block(unsigned s) { return (s & (kBlk - 1u)) <= kBlk - 64u; }
+
+template <class SP>
+constexpr bool kRunAddressing = SP::kBytes >= 4;
+
+// The run descriptor of a wavefront of consecutive cells g0 .. g0+63 (uniform
+// checks; a block-crossing run sends the whole wavefront to per-lane addresses
+// for that direction of access).
+__device__ __forceinline__ Run make_run(unsigned g0, unsigned lo, const int *d) {
+    bool ld = true, st = true;
+#pragma unroll
+    for (int i = 0; i < 18; ++i) {
+        ld = ld && run_in_block(g0 - (unsigned)d[i]);
+        st = st && run_in_block(g0 + (unsigned)d[i]);
+    }
+    return Run{ld, st, g0, lo};
+}
+
 template <class SP, bool BETA, bool NTL>
 __device__ __forceinline__ void cell_load(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
-                                          typename SP::T *f) {
+                                          typename SP::T *f, const Run &run = Run{false, false, 0u, 0u}) {
     const unsigned own = cell_base(g) * SP::kBytes;
     if (BETA) {
+        if (run.ld) {
 #pragma unroll
-        for (int i = 0; i < 18; ++i) {
-            const unsigned idx = wrap ? (unsigned)wrap_idx((int)g - d[i], cells) : g - (unsigned)d[i];
-            f[i] = kld<SP>(pl[opp(i)], cell_base(idx) * SP::kBytes, opp(i));
+            for (int i = 0; i < 18; ++i)
+                f[i] = kld<SP>(pl[opp(i)] + (size_t)cell_base(run.g0 - (unsigned)d[i]) * SP::kBytes, run.lo, opp(i));
+        } else {
+#pragma unroll
+            for (int i = 0; i < 18; ++i) {
+                const unsigned idx = wrap ? (unsigned)wrap_idx((int)g - d[i], cells) : g - (unsigned)d[i];
+                f[i] = kld<SP>(pl[opp(i)], cell_base(idx) * SP::kBytes, opp(i));
+            }
         }
         f[18] = kld<SP>(pl[18], own, 18);
     } else {
@@ -563,16 +600,26 @@ __device__ __forceinline__ void beta_st(gchar *base, unsigned off, int i, typena
 
 // Beta scatters f[i] to slot i of W(g + d_i) (lbm_beta.cl:708-784).  Alpha
 // stores every slot of its cell: f[] already holds slot k's value at k.
+// Beta scatter of population i for the lane's cell g: a uniform base plus the
+// lane offset when every scatter run of the wavefront lies in one block.
+template <class SP, bool NTS>
+__device__ __forceinline__ void beta_scatter(gchar *p, unsigned g, int cells, int di, bool wrap, const Run &run, int i,
+                                             typename SP::T v) {
+    if (run.st) {
+        beta_st<SP, NTS>(p + (size_t)cell_base(run.g0 + (unsigned)di) * SP::kBytes, run.lo, i, v);
+    } else {
+        const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + di, cells) : g + (unsigned)di;
+        beta_st<SP, NTS>(p, cell_base(idx) * SP::kBytes, i, v);
+    }
+}
+
 template <class SP, bool BETA, bool NTS>
 __device__ __forceinline__ void cell_store(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
-                                           const typename SP::T *f) {
+                                           const typename SP::T *f, const Run &run = Run{false, false, 0u, 0u}) {
     const unsigned own = cell_base(g) * SP::kBytes;
     if (BETA) {
 #pragma unroll
-        for (int i = 0; i < 18; ++i) {
-            const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[i], cells) : g + (unsigned)d[i];
-            beta_st<SP, NTS>(pl[i], cell_base(idx) * SP::kBytes, i, f[i]);
-        }
+        for (int i = 0; i < 18; ++i) beta_scatter<SP, NTS>(pl[i], g, cells, d[i], wrap, run, i, f[i]);
         kst<SP, NTS>(pl[18], own, 18, f[18]);
     } else {
 #pragma unroll
@@ -585,14 +632,14 @@ __device__ __forceinline__ void cell_store(gchar *const *pl, unsigned g, int cel
 // stores are kept.
 template <class SP, bool BETA, bool NTS>
 __device__ __forceinline__ void cell_store_part(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
-                                                const typename SP::T *f, unsigned mask, unsigned bit) {
+                                                const typename SP::T *f, unsigned mask, unsigned bit,
+                                                const Run &run = Run{false, false, 0u, 0u}) {
     const unsigned own = cell_base(g) * SP::kBytes;
 #pragma unroll
     for (int k = 0; k < 19; ++k) {
         if (((mask >> k) & 1u) != bit) continue;
         if (BETA && k < 18) {
-            const unsigned idx = wrap ? (unsigned)wrap_idx((int)g + d[k], cells) : g + (unsigned)d[k];
-            beta_st<SP, NTS>(pl[k], cell_base(idx) * SP::kBytes, k, f[k]);
+            beta_scatter<SP, NTS>(pl[k], g, cells, d[k], wrap, run, k, f[k]);
         } else {
             kst<SP, NTS>(pl[k], own, k, f[k]);
         }
@@ -615,15 +662,15 @@ __device__ __forceinline__ void alpha_slots(T *f, bool write_dd) {
 
 template <class SP, bool BETA, bool NTS, bool NTL = false>
 __device__ __forceinline__ void cell_scalar(gchar *const *pl, unsigned g, int cells, const int *d, bool wrap,
-                                            const DomDev *D) {
+                                            const DomDev *D, const Run &run = Run{false, false, 0u, 0u}) {
     using T = typename SP::T;
     const int flag = ((const guchar *)D->flags)[g];
     T f[19];
-    cell_load<SP, BETA, NTL>(pl, g, cells, d, wrap, f);
+    cell_load<SP, BETA, NTL>(pl, g, cells, d, wrap, f, run);
     T vx, vy, vz, dens;
     const bool write_dd = cell_update<T, BETA, SP::kShifted>(f, flag, D, vx, vy, vz, dens);
     if (!BETA) alpha_slots(f, write_dd);
-    cell_store<SP, BETA, NTS>(pl, g, cells, d, wrap, f);  // beta stores every cell
+    cell_store<SP, BETA, NTS>(pl, g, cells, d, wrap, f, run);  // beta stores every cell
     if (flag == kFlagGhost) return;  // lbm_alpha.cl:30-32, lbm_beta.cl:787-788
     store_macros<T>(D, g, cells, vx, vy, vz, dens);
 }
@@ -740,7 +787,9 @@ __device__ __forceinline__ void chain_interior(const DomDev *__restrict__ doms, 
 #pragma unroll
         for (int i = 0; i < 19; ++i) d[i] = 0;
     }
-    cell_scalar<SP, BETA, NTS, NTL>(pl, (unsigned)(gs + (int)(threadIdx.x & 63)), cells, d, wrap, D);
+    Run run{false, false, 0u, 0u};
+    if (BETA && kRunAddressing<SP> && !wrap) run = make_run((unsigned)gs, (threadIdx.x & 63u) * SP::kBytes, d);
+    cell_scalar<SP, BETA, NTS, NTL>(pl, (unsigned)(gs + (int)(threadIdx.x & 63)), cells, d, wrap, D, run);
 }
 
 // One chain tile.  Senders put their face values in LDS; after the barrier a
@@ -818,9 +867,16 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
     const int from = !live ? -1 : x == 0 ? C->left[m] : (x == X - 1 ? C->right[m] : -1);
     const bool side0 = __ballot(from >= 0 && x == 0) != 0, side1 = __ballot(from >= 0 && x == X - 1) != 0;
     const unsigned late = BETA ? (1u << 19) - 1u : (side0 ? D->xmask_a[0] : 0u) | (side1 ? D->xmask_a[1] : 0u);
+    // every lane of the wavefront live: the cells are consecutive (one member's
+    // rows, or one 64-cell run)
+    Run run{false, false, 0u, 0u};
+    if (BETA && kRunAddressing<SP> && !wrap && __ballot(live) == ~0ull) {
+        const unsigned w0 = __builtin_amdgcn_readfirstlane((unsigned)g);
+        run = make_run(w0, ((unsigned)g - w0) * SP::kBytes, d);
+    }
     if (live) {
         flag = ((const guchar *)D->flags)[g];
-        cell_load<SP, BETA, NTL>(pl, (unsigned)g, cells, d, wrap, f);
+        cell_load<SP, BETA, NTL>(pl, (unsigned)g, cells, d, wrap, f, run);
         const bool write_dd = cell_update<T, BETA, SP::kShifted>(f, flag, D, vx, vy, vz, dens);
         if (!BETA) alpha_slots(f, write_dd);
         if (to >= 0) {
@@ -828,7 +884,7 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
             for (int k = 0; k < 19; ++k)
                 if (!BETA || ex(k) == (sside ? -1 : 1)) so[k] = f[k];
         }
-        cell_store_part<SP, BETA, NTS>(pl, (unsigned)g, cells, d, wrap, f, late, 0u);
+        cell_store_part<SP, BETA, NTS>(pl, (unsigned)g, cells, d, wrap, f, late, 0u, run);
     } else if (to >= 0) {
         // frozen sender: alpha's slot values are its own slots, beta's are the
         // slots its scatter would write
@@ -854,7 +910,7 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
                 if (take) f[k] = o[k];
             }
         }
-        cell_store_part<SP, BETA, NTS>(pl, (unsigned)g, cells, d, wrap, f, late, 1u);
+        cell_store_part<SP, BETA, NTS>(pl, (unsigned)g, cells, d, wrap, f, late, 1u, run);
         if (flag != kFlagGhost) store_macros<T>(D, (unsigned)g, cells, vx, vy, vz, dens);
     }
 }
@@ -916,9 +972,15 @@ __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ do
         cells_vector<SP, BETA, N>(pl, (unsigned)g0, cells, d, D);
         return;
     }
+    Run run{false, false, 0u, 0u};
+    if (BETA && kRunAddressing<SP> && N == 1 && !wrap) {
+        // lane 0 holds the wavefront's first cell (lanes past the end have returned)
+        const unsigned w0 = __builtin_amdgcn_readfirstlane((unsigned)g0);
+        if (w0 + 63u < (unsigned)cells) run = make_run(w0, ((unsigned)g0 - w0) * SP::kBytes, d);
+    }
 #pragma unroll
     for (int j = 0; j < N; ++j)
-        if (g0 + j < cells) cell_scalar<SP, BETA, NTS, NTL>(pl, (unsigned)(g0 + j), cells, d, wrap, D);
+        if (g0 + j < cells) cell_scalar<SP, BETA, NTS, NTL>(pl, (unsigned)(g0 + j), cells, d, wrap, D, run);
 }
 
 }  // namespace lbm
