@@ -197,8 +197,12 @@ int lbm_group_set_exchange_mode(lbm_group *g, int mode);
  * line).  Every slot of every member is identical to the copy-based
  * exchange.  Chains whose rows (or, for wide rows, whose 64-cell face runs)
  * do not fit the work-group keep their copies.  lbm_group_sync still
- * copies. */
-int lbm_group_set_fused_x(lbm_group *g, int enable);
+ * copies.  `mode`: 0 = copy launch, 1 = inside the step kernel,
+ * LBM_FUSED_X_AUTO = inside the step kernel for FP32/FP64 storage and the copy
+ * launch for FP16/BF16 (measured faster for the two-byte storages, whose
+ * kernels lose more to the handoff tiles than the copy launch costs). */
+#define LBM_FUSED_X_AUTO 2
+int lbm_group_set_fused_x(lbm_group *g, int mode);
 /* replay the step pair as a hipGraph (1) or launch eagerly (0) */
 int lbm_group_set_graph(lbm_group *g, int enable);
 /* computeNextStep() for all members, `steps` times (src/CController.hpp:385-391) */

@@ -322,6 +322,38 @@ def test_full_size_precision_channel_fused_equals_copies(storage):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("storage", [_capi.STORAGE_F32, _capi.STORAGE_F64, _capi.STORAGE_F16_SHIFTED,
+                                     _capi.STORAGE_BF16_SHIFTED])
+def test_fused_x_auto_picks_per_storage(storage):
+    """LBM_FUSED_X_AUTO: the periodic seam of a channel goes inside the step
+    kernel for FP32/FP64 (no exchange launch) and through the copy launch
+    for FP16/BF16 (one per step); the fields equal both
+    explicit modes bit for bit. Other mode values are rejected."""
+    say(f"case: fused x auto, channel (40,18,16) storage {storage}")
+    from turbulent_lbm_multigpu_amd import precision
+
+    out, launches = [], []
+    for mode in (_capi.FUSED_X_AUTO, 1, 0):
+        s, g, _ = precision.build_channel((40, 18, 16), storage, tau=0.55, gx=2e-5, fill=0.04, rmin=2.0, rmax=3.0,
+                                          kernel_variant=3, fused_x=mode)
+        g.set_timing(True)
+        g.step(8)
+        g.wait()
+        launches.append(g.timing()["exchange_launches"])
+        out.append(s.store(_capi.FIELD_DD_RAW))
+        if mode == 0:
+            with pytest.raises(RuntimeError):
+                g.set_fused_x(3)
+        g.close()
+        s.close()
+    two_byte = storage in (_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED)
+    # one launch per step (a step is one half-step kernel, alpha or beta)
+    assert launches[1] == 0 and launches[2] == 8
+    assert launches[0] == (8 if two_byte else 0)
+    np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
+
+
 @pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
 def test_shifted_storage_error_vs_fp64(storage):
     """FP16/BF16 storage of f - w_i with FP32 collision stays close to FP64."""

@@ -32,6 +32,8 @@ STORAGE_F64 = 1
 STORAGE_F16_SHIFTED = 2
 STORAGE_BF16_SHIFTED = 3
 
+FUSED_X_AUTO = 2  # x faces in the step kernel for FP32/FP64 storage, copy launch for FP16/BF16
+
 FORCING_REFERENCE = 0
 FORCING_BODY = 1
 

@@ -143,7 +143,9 @@ public:
     double perturbation_amplitude = 0.0;
     unsigned long long perturbation_seed = 0;
     int exchange_mode = 0;  // 0: all 19 populations after alpha (reference), 1: the 5 an inner cell reads
-    int fused_x = 1;        // 1: x faces exchanged inside the step kernel (same results as 0, the copy launch)
+    // x faces: 1 inside the step kernel, 0 the copy launch (same results), 2 (LBM_FUSED_X_AUTO) the
+    // faster of the two for the storage: inside the kernel for f32/f64, the copy launch for f16/bf16
+    int fused_x = LBM_FUSED_X_AUTO;
 
     CConfiguration() {}
     explicit CConfiguration(const std::string &file_name) { loadFile(file_name); }

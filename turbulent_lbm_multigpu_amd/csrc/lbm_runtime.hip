@@ -1596,9 +1596,12 @@ extern "C" int lbm_group_set_exchange_mode(lbm_group *g, int mode) {
     return LBM_OK;
 }
 
-extern "C" int lbm_group_set_fused_x(lbm_group *g, int enable) {
+extern "C" int lbm_group_set_fused_x(lbm_group *g, int mode) {
     if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
-    g->fuse_x = enable != 0;
+    if (mode != 0 && mode != 1 && mode != LBM_FUSED_X_AUTO)
+        return gfail(g, LBM_ERR_INVALID_ARGUMENT, "fused-x mode must be 0, 1 or LBM_FUSED_X_AUTO");
+    if (mode == LBM_FUSED_X_AUTO) mode = elem_size(g->storage) >= 4;
+    g->fuse_x = mode != 0;
     g->plan_built = false;
     return LBM_OK;
 }

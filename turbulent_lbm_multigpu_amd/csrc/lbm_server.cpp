@@ -175,7 +175,8 @@ int pool_open(lbm_server *s, int slot) {
     int rc = lbm_group_create_empty(s->cfg.device, kStorages[slot], s->cfg.work_group_size, s->cfg.kernel_variant, &p.g);
     if (rc) return s->fail_lib(rc, "group creation", lbm_last_error());
     rc = lbm_group_set_exchange_mode(p.g, 1);
-    if (!rc) rc = lbm_group_set_fused_x(p.g, 1);  // periodic channel seams inside the step kernel
+    // periodic channel seams inside the step kernel for FP32/FP64, copy launch for FP16/BF16
+    if (!rc) rc = lbm_group_set_fused_x(p.g, LBM_FUSED_X_AUTO);
     if (rc) return s->fail_lib(rc, "exchange mode", lbm_group_error(p.g));
     void *st = nullptr;
     lbm_group_get_stream(p.g, &st);

@@ -28,10 +28,10 @@ STORAGES = {"f64": _capi.STORAGE_F64, "f32": _capi.STORAGE_F32, "f16": _capi.STO
 
 
 def build_channel(dims, storage, *, tau=0.51, cs=0.17, gx=1e-6, sphere_seed=20260101, fill=0.05, rmin=3.0,
-                  rmax=8.0, amp=0.0, seed=0, kernel_variant=0, work_group_size=256, fused_x=True):
+                  rmax=8.0, amp=0.0, seed=0, kernel_variant=0, work_group_size=256, fused_x=_capi.FUSED_X_AUTO):
     """One periodic channel in a group of one subdomain whose two x comms point
     at itself (the periodic seam is a ghost-layer exchange, done inside the
-    step kernel with fused_x)."""
+    step kernel with fused_x: 1, or _capi.FUSED_X_AUTO for FP32/FP64)."""
     sub = dec.decompose(dims, (1, 1, 1), periodic=(True, False, False))[0]
     bc = [sub.bc[0][0], sub.bc[0][1], sub.bc[1][0], sub.bc[1][1], sub.bc[2][0], sub.bc[2][1]]
     s = LbmSolver(dims, bc, inv_tau=1.0 / tau, tau=tau, smagorinsky_cs=cs, gravitation=(gx, 0.0, 0.0),
@@ -97,10 +97,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--storages", nargs="+", default=["f64", "f32", "f16", "bf16"])
     ap.add_argument("--variant", type=int, default=3, help="kernel variant (3: non-temporal loads and stores)")
-    ap.add_argument("--fused-x", type=int, default=1, choices=[0, 1],
-                    help="periodic x faces inside the step kernel (1) or by a copy launch (0)")
+    ap.add_argument("--fused-x", type=int, default=_capi.FUSED_X_AUTO, choices=[0, 1, _capi.FUSED_X_AUTO],
+                    help="periodic x faces inside the step kernel (1), by a copy launch (0), or "
+                         "per storage (2: in the kernel for f32/f64, copy launch for f16/bf16)")
     a = ap.parse_args()
-    res = run(tuple(a.dims), a.steps, tuple(a.storages), kernel_variant=a.variant, fused_x=bool(a.fused_x))
+    res = run(tuple(a.dims), a.steps, tuple(a.storages), kernel_variant=a.variant, fused_x=a.fused_x)
     print(json.dumps({"workload": f"LES channel {a.dims[0]}x{a.dims[1]}x{a.dims[2]}, periodic x, g_x 1e-6, "
                                   f"tau 0.51, Cs 0.17, 5% seeded spheres r 3-8, {a.steps} steps",
                       "results": res}))

@@ -307,9 +307,10 @@ class LbmGroup:
     def set_exchange_mode(self, mode: int):
         self._check(self._lib.lbm_group_set_exchange_mode(self._h, int(mode)))
 
-    def set_fused_x(self, enable: bool):
-        """Exchange same-sized x faces inside the step kernel."""
-        self._check(self._lib.lbm_group_set_fused_x(self._h, int(bool(enable))))
+    def set_fused_x(self, mode):
+        """Exchange same-sized x faces inside the step kernel (True / 1), by
+        the copy launch (False / 0), or per storage (_capi.FUSED_X_AUTO)."""
+        self._check(self._lib.lbm_group_set_fused_x(self._h, int(mode)))
 
     def set_graph(self, enable: bool):
         self._check(self._lib.lbm_group_set_graph(self._h, int(bool(enable))))
