@@ -84,7 +84,10 @@ typedef struct lbm_solver_config {
     int store_density;      /* kernels write the density field (STORE_DENSITY) */
     int work_group_size;    /* threads per work-group of the step kernels: 64..256; 0 = 256 */
     int device;             /* HIP device ordinal */
-    int kernel_variant;     /* tuning bits: 1 non-temporal loads, 2 non-temporal stores, 4 no XCD tile map */
+    int kernel_variant;     /* tuning bits: 1 non-temporal loads, 2 non-temporal stores, 4 no XCD tile map,
+                               8 / 16 two / four x-consecutive cells per work-item; supported: 0-3, 8, 11
+                               (two cells with the non-temporal policy; two-byte storages keep the pair
+                               packed in one 32-bit word) and 16, each optionally | 4 */
     int lds_per_block;      /* bytes of LDS reserved per work-group (caps residency); 0 = none */
 } lbm_solver_config;
 
@@ -223,9 +226,8 @@ int lbm_group_get_timing(lbm_group *g, double *step_kernel_ms, long long *step_k
 int lbm_group_set_launch(lbm_group *g, int work_group_size, int kernel_variant);
 int lbm_group_get_launch(const lbm_group *g, int *work_group_size, int *kernel_variant);
 /* autotuner (new): times `pairs` step pairs for each work-group size in
- * {64,128,256} x kernel variant in {0,3,8,16} (1 cell per lane with the
- * default or the non-temporal cache policy, 2 or 4 cells per lane), keeps the
- * fastest; the timed steps are real steps of the simulation (24 * (pairs + 1)
+ * {64,128,256} x kernel variant in {0,3,8,11} (1 or 2 cells per lane, each
+ * with the default or the non-temporal cache policy), keeps the fastest; the timed steps are real steps of the simulation (24 * (pairs + 1)
  * of them, plus one if the counter was odd) */
 int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step);
 /* the same averages split into beta and alpha step kernels */

@@ -527,12 +527,13 @@ def test_fused_x_periodic_channel(dims):
 @pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
 @pytest.mark.parametrize("dims", [(20, 18, 16), (21, 17, 19), (64, 12, 10)])
 def test_two_byte_storage_launch_shapes_identical(storage, dims):
-    """FP16/BF16 storage: 1, 2 and 4 cells per lane (vector accesses, element
-    accesses where a shifted run straddles a layout block), work-groups
+    """FP16/BF16 storage: 1, 2 and 4 cells per lane (packed 32-bit words at
+    any 2-byte alignment, element accesses where a pair straddles a layout
+    block), with and without the non-temporal policy, work-groups
     64/128/256, give bit-identical states."""
     say(f"case: two-byte storage {storage} dims {dims} launch shapes")
     out = []
-    for variant, wg in ((0, 256), (8, 256), (8, 64), (16, 128)):
+    for variant, wg in ((0, 256), (3, 256), (8, 256), (8, 64), (11, 256), (11, 128), (16, 128)):
         s = LbmSolver(dims, (1,) * 6, inv_tau=1 / 0.53, tau=0.53, smagorinsky_cs=0.17, lid_velocity=0.06,
                       storage=storage, kernel_variant=variant, work_group_size=wg)
         s.init_perturbed(1e-3, 17)
@@ -547,18 +548,20 @@ def test_two_byte_storage_launch_shapes_identical(storage, dims):
 
 def test_two_byte_channel_pairs_with_fused_x():
     """Vector lanes next to fused periodic x faces (edge lanes take the
-    element path): FP16 channel, 2 cells per lane == 1 cell per lane."""
+    element path): FP16 channel, 2 cells per lane == 1 cell per lane, with
+    the x faces in the kernel or by the copy launch."""
     say("case: fp16 channel pairs + fused x")
     from turbulent_lbm_multigpu_amd import precision
 
     out = []
-    for variant in (0, 8):
+    for variant, fused in ((0, True), (8, True), (11, True), (11, False)):
         s, g, _ = precision.build_channel((32, 18, 16), _capi.STORAGE_F16_SHIFTED, tau=0.55, gx=2e-5, fill=0.04,
                                           rmin=2.0, rmax=3.0, amp=1e-3, seed=9, kernel_variant=variant)
-        g.set_fused_x(True)
+        g.set_fused_x(fused)
         g.step(15)
         g.wait()
         out.append(gpu_dd(s))
         g.close()
         s.close()
-    np.testing.assert_array_equal(out[0], out[1])
+    for other in out[1:]:
+        np.testing.assert_array_equal(out[0], other)

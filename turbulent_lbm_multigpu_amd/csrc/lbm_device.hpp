@@ -753,6 +753,87 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
         if (flag[j] != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx[j], vy[j], vz[j], dn[j]);
 }
 
+// Cells s and s+1 of population base p as one 32-bit word (cell s in the low
+// half), for the two-byte storages: one access when both lie in one layout
+// block (at any 2-byte alignment: gfx950 serves unaligned dword accesses),
+// two element accesses at a block's last cell.
+typedef __attribute__((address_space(1))) unsigned __attribute__((aligned(2))) gpair;
+__device__ __forceinline__ unsigned pair_ld(const gchar *p, unsigned s) {
+    if ((s & (kBlk - 1u)) != kBlk - 1u) return *(const gpair *)(p + cell_base(s) * 2u);
+    return (unsigned)raw_ld<false, unsigned short>(p, cell_base(s) * 2u) |
+           ((unsigned)raw_ld<false, unsigned short>(p, cell_base(s + 1u) * 2u) << 16);
+}
+template <bool NT>
+__device__ __forceinline__ void pair_st(gchar *p, unsigned s, unsigned v) {
+    if ((s & (kBlk - 1u)) != kBlk - 1u) {
+        if (NT && (s & 1u) == 0u) {
+            raw_st<true, unsigned>(p, cell_base(s) * 2u, v);
+        } else {
+            *(gpair *)(p + cell_base(s) * 2u) = v;
+        }
+        return;
+    }
+    raw_st<NT, unsigned short>(p, cell_base(s) * 2u, (unsigned short)v);
+    raw_st<NT, unsigned short>(p, cell_base(s + 1u) * 2u, (unsigned short)(v >> 16));
+}
+
+// Two x-consecutive cells g0, g0+1 per work-item in a two-byte storage (g0
+// even, away from the domain ends): every population access moves one 32-bit
+// word holding both cells, and the words stay packed in registers.  The two
+// cells collide one after the other, each unpacking its 19 halves and packing
+// its results back in place, so only one cell's values are live as floats
+// (cells_vector keeps all 2 x 19).  Same conversions, arithmetic and stored
+// values as two calls of cell_scalar.
+template <class SP, bool BETA, bool NTS, bool NTL>
+__device__ __forceinline__ void cells_pair2(gchar *const *pl, unsigned g0, int cells, const int *d, const DomDev *D) {
+    static_assert(SP::kBytes == 2, "packed pairs hold two-byte storages");
+    using T = typename SP::T;
+    const unsigned fl = raw_ld<false, unsigned short>((const gchar *)D->flags, g0);
+    const unsigned own = cell_base(g0) * 2u;
+    unsigned P[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        if (BETA && i < 18) P[i] = pair_ld(pl[opp(i)], g0 - (unsigned)d[i]);
+        else P[i] = raw_ld<NTL && !BETA, unsigned>(pl[i], own);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int flag = (int)((fl >> (8 * j)) & 0xffu);
+        T f[19], vx, vy, vz, dn;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) f[i] = kin<SP>((typename SP::U)(P[i] >> (16 * j)), i);
+        const bool wr = cell_update<T, BETA, SP::kShifted>(f, flag, D, vx, vy, vz, dn);
+        if (wr) {
+            // beta: slot i of the scatter gets f[i]; alpha: slot k of the cell gets f[opp(k)]
+#pragma unroll
+            for (int k = 0; k < 19; ++k) {
+                const unsigned h = kout<SP>(BETA ? f[k] : f[opp(k)], k);
+                P[k] = j ? ((P[k] & 0xffffu) | (h << 16)) : ((P[k] & 0xffff0000u) | h);
+            }
+        }
+        // the macroscopic outputs go out at once: held for the end, they stay
+        // live through the other cell's collision
+        if (flag != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx, vy, vz, dn);
+    }
+    // The store addresses start from a fresh copy of the cell index: reusing
+    // the loads' address arithmetic, the compiler keeps 19 64-bit per-lane
+    // addresses live through both collisions (~40 VGPRs).
+    unsigned gs = g0;
+    asm volatile("" : "+v"(gs));
+    const unsigned own_st = cell_base(gs) * 2u;
+    if (BETA) {
+#pragma unroll
+        for (int i = 0; i < 18; ++i) {
+            if (NTS && ex(i) == 0) pair_st<true>(pl[i], gs + (unsigned)d[i], P[i]);
+            else pair_st<false>(pl[i], gs + (unsigned)d[i], P[i]);
+        }
+        raw_st<NTS, unsigned>(pl[18], own_st, P[18]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 19; ++k) raw_st<NTS, unsigned>(pl[k], own_st, P[k]);
+    }
+}
+
 // An interior tile of a run-tiled chain: the `pass`-th share of the interior
 // runs of the rows of row tile `rt` (columns 64 .. X-3 of every member, one
 // 64-cell run per wavefront).  No column of a joined face: no LDS, no
@@ -970,7 +1051,8 @@ __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ do
         for (int i = 0; i < 19; ++i) d[i] = 0;
     }
     if (N > 1 && !wrap && g0 + N <= cells) {
-        cells_vector<SP, BETA, N>(pl, (unsigned)g0, cells, d, D);
+        if constexpr (N == 2 && SP::kBytes == 2) cells_pair2<SP, BETA, NTS, NTL>(pl, (unsigned)g0, cells, d, D);
+        else cells_vector<SP, BETA, N>(pl, (unsigned)g0, cells, d, D);
         return;
     }
     Run run{false, false, 0u, 0u};
@@ -979,7 +1061,9 @@ __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ do
         const unsigned w0 = __builtin_amdgcn_readfirstlane((unsigned)g0);
         if (w0 + 63u < (unsigned)cells) run = make_run(w0, ((unsigned)g0 - w0) * SP::kBytes, d);
     }
-#pragma unroll
+    // one cell at a time: interleaving N copies of the scalar update would set
+    // the kernel's register count for a path only the domain ends take
+#pragma unroll 1
     for (int j = 0; j < N; ++j)
         if (g0 + j < cells) cell_scalar<SP, BETA, NTS, NTL>(pl, (unsigned)(g0 + j), cells, d, wrap, D, run);
 }

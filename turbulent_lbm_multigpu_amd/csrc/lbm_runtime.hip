@@ -507,6 +507,7 @@ hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int 
                 case 2: go(b, std::integral_constant<int, 2>()); break;
                 case 3: go(b, std::integral_constant<int, 3>()); break;
                 case 8: go(b, std::integral_constant<int, 8>()); break;
+                case 11: go(b, std::integral_constant<int, 11>()); break;
                 case 16: go(b, std::integral_constant<int, 16>()); break;
                 default: go(b, std::integral_constant<int, 0>()); break;
             }
@@ -657,7 +658,7 @@ static int validate_config(const lbm_solver_config &c) {
     if (c.smagorinsky_cs < 0.0 || (c.smagorinsky_cs > 0.0 && !(c.tau > 0.5)))
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "LES needs smagorinsky_cs >= 0 and tau > 0.5");
     const int v = c.kernel_variant & ~4;
-    if (c.kernel_variant < 0 || (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 16) || c.lds_per_block < 0 ||
+    if (c.kernel_variant < 0 || (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 11 && v != 16) || c.lds_per_block < 0 ||
         c.lds_per_block > 65536)
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
                       "kernel_variant must be one of 0, 1, 2, 3, 8, 16 (optionally | 4); lds_per_block in [0, 65536]");
@@ -1453,7 +1454,7 @@ extern "C" int lbm_group_create_empty(int device, int storage, int work_group_si
     *out = nullptr;
     const int v = kernel_variant & ~4;
     if (storage < 0 || storage > 3 || work_group_size < 64 || work_group_size > 256 || work_group_size % 64 != 0 ||
-        kernel_variant < 0 || (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 16))
+        kernel_variant < 0 || (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 11 && v != 16))
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad storage or launch shape");
     lbm_group *g = new lbm_group();
     g->storage = storage;
@@ -1881,7 +1882,7 @@ extern "C" int lbm_group_set_launch(lbm_group *g, int work_group_size, int kerne
     if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
     const int v = kernel_variant & ~4;
     if (work_group_size < 64 || work_group_size > 256 || work_group_size % 64 != 0 || kernel_variant < 0 ||
-        (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 16))
+        (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 11 && v != 16))
         return gfail(g, LBM_ERR_INVALID_ARGUMENT, "unsupported launch shape");
     GCHK(hipStreamSynchronize(g->stream));
     g->wg = work_group_size;
@@ -1900,9 +1901,9 @@ extern "C" int lbm_group_get_launch(const lbm_group *g, int *work_group_size, in
 }
 
 // Autotuner: times `pairs` beta+alpha step pairs (with their exchanges) for
-// every work-group size in {64, 128, 256} and every variant in {0, 3, 8, 16}
-// (1 cell per lane with the default or the non-temporal cache policy, 2 or 4
-// cells per lane) and keeps the fastest.  The steps it runs are
+// every work-group size in {64, 128, 256} and every variant in {0, 3, 8, 11}
+// (1 or 2 cells per lane, each with the default or the non-temporal cache
+// policy) and keeps the fastest.  The steps it runs are
 // ordinary simulation steps: every launch shape computes the same
 // arithmetic, so the state is the one the same number of steps gives with
 // any shape.
@@ -1919,10 +1920,17 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
     }
     if (g->ntiles == 0) return gfail(g, LBM_ERR_STATE, "autotune needs at least one member");
     const int wgs[3] = {64, 128, 256};
-    const int vars[4] = {0, 3, 8, 16};
+    const int vars[4] = {0, 3, 8, 11};
     double best = 1e300;
     int bw = g->wg, bv = g->variant;
-    hipEvent_t e0, e1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    struct Events {
+        hipEvent_t *a, *b;
+        ~Events() {
+            if (*a) (void)hipEventDestroy(*a);
+            if (*b) (void)hipEventDestroy(*b);
+        }
+    } events_guard{&e0, &e1};
     GCHK(hipEventCreate(&e0));
     GCHK(hipEventCreate(&e1));
     for (int wi = 0; wi < 3; ++wi)
@@ -1951,8 +1959,6 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
                 bv = vars[vi] | (g->variant & 4);
             }
         }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     int rc = lbm_group_set_launch(g, bw, bv);
     if (!rc) rc = group_prepare(g);
     if (rc) return rc;

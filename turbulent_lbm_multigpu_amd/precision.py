@@ -25,6 +25,10 @@ from .solver import LbmGroup, LbmSolver
 
 STORAGES = {"f64": _capi.STORAGE_F64, "f32": _capi.STORAGE_F32, "f16": _capi.STORAGE_F16_SHIFTED,
             "bf16": _capi.STORAGE_BF16_SHIFTED}
+# fastest measured kernel variant per storage on this channel: non-temporal
+# accesses, one cell per work-item for f64/f32, two for the two-byte storages
+# (their pair of cells moves as one 32-bit word)
+DEFAULT_VARIANT = {"f64": 3, "f32": 3, "f16": 11, "bf16": 11}
 
 
 def build_channel(dims, storage, *, tau=0.51, cs=0.17, gx=1e-6, sphere_seed=20260101, fill=0.05, rmin=3.0,
@@ -59,11 +63,13 @@ def fields(s: LbmSolver):
     return rho, u, fluid
 
 
-def run(dims=(384, 192, 192), steps=10000, storages=("f64", "f32", "f16", "bf16"), **kw):
+def run(dims=(384, 192, 192), steps=10000, storages=("f64", "f32", "f16", "bf16"), kernel_variant=None, **kw):
+    """kernel_variant None: DEFAULT_VARIANT of each storage."""
     out = {}
     ref = None
     for name in storages:
-        s, g, filled = build_channel(dims, STORAGES[name], **kw)
+        variant = DEFAULT_VARIANT[name] if kernel_variant is None else kernel_variant
+        s, g, filled = build_channel(dims, STORAGES[name], kernel_variant=variant, **kw)
         m0, k0 = s.mass_energy()
         g.step(2)
         g.wait()
@@ -73,7 +79,8 @@ def run(dims=(384, 192, 192), steps=10000, storages=("f64", "f32", "f16", "bf16"
         el = time.perf_counter() - t0
         m1, k1 = s.mass_energy()
         rho, u, fluid = fields(s)
-        r = {"mlups": round(np.prod(dims) * (steps - 2) / el / 1e6, 1), "mass_start": m0, "mass_end": m1,
+        r = {"mlups": round(np.prod(dims) * (steps - 2) / el / 1e6, 1), "kernel_variant": variant,
+             "mass_start": m0, "mass_end": m1,
              "kinetic_energy": k1, "obstacle_fill": round(filled, 4)}
         if name == "f64":
             ref = (rho, u, fluid, m1, k1)
@@ -96,7 +103,8 @@ def main():
     ap.add_argument("--dims", type=int, nargs=3, default=[384, 192, 192])
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--storages", nargs="+", default=["f64", "f32", "f16", "bf16"])
-    ap.add_argument("--variant", type=int, default=3, help="kernel variant (3: non-temporal loads and stores)")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="kernel variant for every storage (default: 3 for f64/f32, 11 for f16/bf16)")
     ap.add_argument("--fused-x", type=int, default=_capi.FUSED_X_AUTO, choices=[0, 1, _capi.FUSED_X_AUTO],
                     help="periodic x faces inside the step kernel (1), by a copy launch (0), or "
                          "per storage (2: in the kernel for f32/f64, copy launch for f16/bf16)")
