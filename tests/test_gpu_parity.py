@@ -322,6 +322,26 @@ def test_full_size_precision_channel_fused_equals_copies(storage):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
+def test_full_size_precision_channel_packed_pairs(storage):
+    """The precision workload at its full size with the two-byte storages'
+    default launch (two cells per lane as packed 32-bit words, x faces by the
+    copy launch) against one cell per lane with the x faces in the kernel:
+    every stored bit identical after 7 steps."""
+    say(f"case: full-size channel 384x192x192 storage {storage}, packed pairs vs one cell per lane")
+    from turbulent_lbm_multigpu_amd import precision
+
+    out = []
+    for variant, fused in ((11, 0), (3, 1)):
+        s, g, _ = precision.build_channel((384, 192, 192), storage, kernel_variant=variant, fused_x=fused)
+        g.step(7)
+        g.wait()
+        out.append(s.store(_capi.FIELD_DD_RAW))
+        g.close()
+        s.close()
+    np.testing.assert_array_equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("storage", [_capi.STORAGE_F32, _capi.STORAGE_F64, _capi.STORAGE_F16_SHIFTED,
                                      _capi.STORAGE_BF16_SHIFTED])
 def test_fused_x_auto_picks_per_storage(storage):
