@@ -722,35 +722,35 @@ __device__ __forceinline__ void cells_vector(gchar *const *pl, unsigned g0, int 
         if (BETA && i < 18) vld_any<SP, N>(pl[src], g0 - (unsigned)d[i], src, F[i]);
         else vld<SP, N>(pl[src], own, src, F[i]);
     }
-    T vx[N], vy[N], vz[N], dn[N];
-    bool wr[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        T f[19];
+        T f[19], vx, vy, vz, dn;
 #pragma unroll
         for (int i = 0; i < 19; ++i) f[i] = F[i][j];
-        wr[j] = cell_update<T, BETA, SP::kShifted>(f, flag[j], D, vx[j], vy[j], vz[j], dn[j]);
+        const bool wr = cell_update<T, BETA, SP::kShifted>(f, flag[j], D, vx, vy, vz, dn);
+        // slot values in place: beta scatters f[i] to slot i, alpha stores
+        // f[opp(k)] in slot k (see cell_scalar); the loaded values where the
+        // cell did not collide
 #pragma unroll
-        for (int i = 0; i < 19; ++i) F[i][j] = wr[j] ? f[i] : F[i][j];
+        for (int k = 0; k < 19; ++k) F[k][j] = wr ? (BETA ? f[k] : f[opp(k)]) : F[k][j];
+        // the macroscopic outputs go out at once instead of staying live
+        // through the other cells' collisions
+        if (flag[j] != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx, vy, vz, dn);
     }
+    // Store addresses from a fresh copy of the cell index: reusing the loads'
+    // address arithmetic, the compiler keeps 64-bit per-lane addresses live
+    // through the collisions (FP32 two cells per lane: 108 -> 79 VGPRs in beta).
+    unsigned gs = g0;
+    asm volatile("" : "+v"(gs));
+    const unsigned own_st = cell_base(gs) * es;
     if (BETA) {
 #pragma unroll
-        for (int i = 0; i < 18; ++i) vst_any<SP, N>(pl[i], g0 + (unsigned)d[i], i, F[i]);
-        vst<SP, N>(pl[18], own, 18, F[18]);
+        for (int i = 0; i < 18; ++i) vst_any<SP, N>(pl[i], gs + (unsigned)d[i], i, F[i]);
+        vst<SP, N>(pl[18], own_st, 18, F[18]);
     } else {
-        // every slot stored: f[opp(k)] where the cell collided, else the
-        // loaded value (see cell_scalar)
 #pragma unroll
-        for (int k = 0; k < 19; ++k) {
-            T v[N];
-#pragma unroll
-            for (int j = 0; j < N; ++j) v[j] = wr[j] ? F[opp(k)][j] : F[k][j];
-            vst<SP, N>(pl[k], own, k, v);
-        }
+        for (int k = 0; k < 19; ++k) vst<SP, N>(pl[k], own_st, k, F[k]);
     }
-#pragma unroll
-    for (int j = 0; j < N; ++j)
-        if (flag[j] != kFlagGhost) store_macros<T>(D, g0 + j, cells, vx[j], vy[j], vz[j], dn[j]);
 }
 
 // Cells s and s+1 of population base p as one 32-bit word (cell s in the low
