@@ -41,6 +41,11 @@ def test_version_and_defaults():
     ("bc", (1, 1, 1, 1, 1, 300), _capi.LBM_ERR_INVALID_ARGUMENT),
     ("storage", 9, _capi.LBM_ERR_INVALID_ARGUMENT),
     ("work_group_size", 100, _capi.LBM_ERR_INVALID_ARGUMENT),
+    ("kernel_variant", 10, _capi.LBM_ERR_INVALID_ARGUMENT),   # variants: 0-3, 8, 11, 16 (each | 4)
+    ("kernel_variant", 13, _capi.LBM_ERR_INVALID_ARGUMENT),   # 9 | 4
+    ("kernel_variant", 9, _capi.LBM_ERR_INVALID_ARGUMENT),
+    ("kernel_variant", 24, _capi.LBM_ERR_INVALID_ARGUMENT),
+    ("kernel_variant", -1, _capi.LBM_ERR_INVALID_ARGUMENT),
     ("inv_tau", -1.0, _capi.LBM_ERR_PARAMETRIZATION),
 ])
 def test_create_rejects_bad_config_without_device(field, value, code):
