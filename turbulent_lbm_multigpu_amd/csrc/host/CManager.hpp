@@ -145,10 +145,12 @@ public:
                     throw "lbm_group_add_comm failed";
             }
         // ghost layers start as copies of their owners (all 19 populations)
-        lbm_group_set_exchange_mode(_group, 0);
-        lbm_group_sync(_group, 0);
-        lbm_group_set_exchange_mode(_group, cfg->exchange_mode);
-        lbm_group_set_fused_x(_group, cfg->fused_x);
+        if (lbm_group_set_exchange_mode(_group, 0) != LBM_OK || lbm_group_sync(_group, 0) != LBM_OK)
+            throw "CManager: initial ghost-layer exchange failed";
+        if (lbm_group_set_exchange_mode(_group, cfg->exchange_mode) != LBM_OK)
+            throw "CManager: <exchange-mode> must be 0 or 1";
+        if (lbm_group_set_fused_x(_group, cfg->fused_x) != LBM_OK)
+            throw "CManager: <fused-x> must be 0, 1 or 2";
         for (auto *ctl : _controllers) ctl->attachGroup(_group);
     }
 
