@@ -1787,36 +1787,43 @@ static std::vector<long long> group_shape(const lbm_group *g) {
     return v;
 }
 
+// A start/stop event pair for the timing of one launch, owned by g->events.
+static hipError_t group_event_pair(lbm_group *g, int kind, hipEvent_t *a, hipEvent_t *b) {
+    hipError_t e = hipEventCreate(a);
+    if (e != hipSuccess) return e;
+    e = hipEventCreate(b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(*a);
+        return e;
+    }
+    g->events.push_back({*a, *b, kind});
+    return hipSuccess;
+}
+
 static int group_launch_half(lbm_group *g, bool beta, bool timed) {
     if (g->ntiles == 0) return LBM_OK;
+    // timing events belong to g->events from creation on, so every exit
+    // path leaves them to drain_events / lbm_group_destroy
     hipEvent_t ea = nullptr, eb = nullptr;
     if (timed) {
-        GCHK(hipEventCreate(&ea));
-        GCHK(hipEventCreate(&eb));
+        GCHK(group_event_pair(g, beta ? 0 : 2, &ea, &eb));
         GCHK(hipEventRecord(ea, g->stream));
     }
     GCHK(launch_step_kernel(g->storage, beta, g->variant, g->lds, g->ntiles, g->wg, g->d_descs, g->d_tiles,
                             g->stream, g->d_ctr, g->d_chains));
-    if (timed) {
-        GCHK(hipEventRecord(eb, g->stream));
-        g->events.push_back({ea, eb, beta ? 0 : 2});
-    }
+    if (timed) GCHK(hipEventRecord(eb, g->stream));
     // the kernel just run was beta -> syncBeta, alpha -> syncAlpha (src/CController.hpp:385-391)
     const int k = beta ? 1 : 0;
     for (int p = 0; p < 3; ++p) {
         if (!g->nops[k][p]) continue;
         hipEvent_t xa = nullptr, xb = nullptr;
         if (timed) {
-            GCHK(hipEventCreate(&xa));
-            GCHK(hipEventCreate(&xb));
+            GCHK(group_event_pair(g, 1, &xa, &xb));
             GCHK(hipEventRecord(xa, g->stream));
         }
         GCHK(exchange_launch(g->storage, g->d_ops[k][p], g->nops[k][p], g->nblocks[k][p], g->stream,
                              g->d_ctr + (beta ? 1 : 0)));
-        if (timed) {
-            GCHK(hipEventRecord(xb, g->stream));
-            g->events.push_back({xa, xb, 1});
-        }
+        if (timed) GCHK(hipEventRecord(xb, g->stream));
     }
     return LBM_OK;
 }
@@ -2008,17 +2015,26 @@ extern "C" int lbm_group_set_timing(lbm_group *g, int enable) {
     return LBM_OK;
 }
 
+// Folds the recorded launch times into the averages and destroys every event
+// pair, also when one of them cannot be read (a launch that failed between
+// its two records); the first such error is reported.
 static int drain_events(lbm_group *g) {
-    GCHK(hipStreamSynchronize(g->stream));
+    const hipError_t sync = hipStreamSynchronize(g->stream);
+    hipError_t first = sync;
     for (auto &e : g->events) {
         float ms = 0.f;
-        GCHK(hipEventElapsedTime(&ms, e.a, e.b));
-        g->acc_ms[e.kind] += ms;
-        g->acc_n[e.kind] += 1;
+        const hipError_t te = sync == hipSuccess ? hipEventElapsedTime(&ms, e.a, e.b) : sync;
+        if (te == hipSuccess) {
+            g->acc_ms[e.kind] += ms;
+            g->acc_n[e.kind] += 1;
+        } else if (first == hipSuccess) {
+            first = te;
+        }
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
     g->events.clear();
+    if (first != hipSuccess) return gfail(g, LBM_ERR_DEVICE, std::string("launch timing: ") + hipGetErrorString(first));
     return LBM_OK;
 }
 
