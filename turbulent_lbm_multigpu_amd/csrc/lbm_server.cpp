@@ -227,6 +227,22 @@ int case_setup(lbm_server *s, Case *c) {
     return LBM_OK;
 }
 
+// The case leaves the server as failed: its solver and arena block go, its
+// error is the server's last one.
+void case_fail(lbm_server *s, Case *c) {
+    if (c->solver) lbm_solver_destroy(c->solver);
+    c->solver = nullptr;
+    s->arena.release(c->offset, c->bytes);
+    s->stats.bytes_in_use -= align256(c->bytes);
+    c->state = LBM_CASE_FAILED;
+    c->error = s->err;
+    c->done_s = s->now();
+    s->done_order.push_back(c->id);
+    s->stats.failed++;
+}
+
+// Joins the case's storage pool; on failure the group is left as it was (the
+// member and the comms naming it are removed again).
 int pool_join(lbm_server *s, Case *c) {
     const int slot = storage_slot(c->c.storage);
     int rc = pool_open(s, slot);
@@ -235,7 +251,7 @@ int pool_join(lbm_server *s, Case *c) {
     int idx = -1;
     rc = lbm_group_add_member(p.g, c->solver, &idx);
     if (rc) return s->fail_lib(rc, "group join", lbm_group_error(p.g));
-    p.members.push_back(c);
+    const char *step = nullptr;
     if (c->c.kind == LBM_CASE_CHANNEL) {
         // periodic x: the case is its own neighbour across both x faces
         const int X = c->c.dims[0], Y = c->c.dims[1], Z = c->c.dims[2];
@@ -244,10 +260,18 @@ int pool_join(lbm_server *s, Case *c) {
         const int so1[3] = {X - 2, 0, 0}, ro1[3] = {X - 1, 0, 0}, n1[3] = {-1, 0, 0};
         rc = lbm_group_add_comm(p.g, idx, idx, so0, ro0, size, n0);
         if (!rc) rc = lbm_group_add_comm(p.g, idx, idx, so1, ro1, size, n1);
-        if (rc) return s->fail_lib(rc, "periodic comms", lbm_group_error(p.g));
+        if (rc) step = "periodic comms";
     }
-    rc = lbm_group_set_member_stop(p.g, idx, c->c.steps);
-    if (rc) return s->fail_lib(rc, "member stop", lbm_group_error(p.g));
+    if (!rc) {
+        rc = lbm_group_set_member_stop(p.g, idx, c->c.steps);
+        if (rc) step = "member stop";
+    }
+    if (rc) {
+        const int code = s->fail_lib(rc, step, lbm_group_error(p.g));
+        (void)lbm_group_remove_member(p.g, idx);
+        return code;
+    }
+    p.members.push_back(c);
     c->state = LBM_CASE_RUNNING;
     return LBM_OK;
 }
@@ -313,15 +337,7 @@ int admit(lbm_server *s) {
         if (rc) {
             // the case fails on its own: its block returns to the arena and
             // the server goes on with the others
-            if (c->solver) lbm_solver_destroy(c->solver);
-            c->solver = nullptr;
-            s->arena.release(c->offset, c->bytes);
-            s->stats.bytes_in_use -= align256(c->bytes);
-            c->state = LBM_CASE_FAILED;
-            c->error = s->err;
-            c->done_s = s->now();
-            s->done_order.push_back(c->id);
-            s->stats.failed++;
+            case_fail(s, c);
             continue;
         }
         c->admit_s = s->now();
@@ -483,16 +499,17 @@ extern "C" int lbm_server_progress(lbm_server *s, int *completed) {
             s->retiring.push_back(c);
         }
     }
-    // 3. cases initialised during the last chunk join
-    for (Case *c : s->ready) {
+    // 3. cases initialised during the last chunk join; one that cannot join
+    // fails on its own, as in admission
+    std::vector<Case *> ready;
+    ready.swap(s->ready);
+    for (Case *c : ready) {
         if (c->c.steps == 0) {
             s->retiring.push_back(c);
             continue;
         }
-        int rc = pool_join(s, c);
-        if (rc) return rc;
+        if (pool_join(s, c) != LBM_OK) case_fail(s, c);
     }
-    s->ready.clear();
     // 4. the next chunk, one graph-replayed step sequence per storage
     for (Pool &p : s->pools) {
         if (p.members.empty()) continue;
