@@ -542,12 +542,12 @@ extern "C" int lbm_server_progress(lbm_server *s, int *completed) {
         s->inflight = true;
         s->stats.chunks++;
     }
-    // 5. while it runs: results of the retired cases, then admissions
-    for (Case *c : s->retiring) {
-        int rc = case_readback(s, c);
-        if (rc) return rc;
-    }
-    s->retiring.clear();
+    // 5. while it runs: results of the retired cases (a case whose readback
+    // fails fails on its own), then admissions
+    std::vector<Case *> retiring;
+    retiring.swap(s->retiring);
+    for (Case *c : retiring)
+        if (case_readback(s, c) != LBM_OK) case_fail(s, c);
     int rc = admit(s);
     if (rc) return rc;
     if (completed) *completed = (int)(s->stats.completed - done_before);
