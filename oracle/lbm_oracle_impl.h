@@ -148,6 +148,19 @@ static REAL FN(oracle_les_inv_tau)(const REAL *f, REAL rho, REAL jx, REAL jy, RE
     return ((REAL)1.0) / tau_eff;
 }
 
+/* Exported for the tests: the Smagorinsky rate of one cell's populations,
+ * with rho and j summed here in index order. */
+REAL FN(oracle_les_rate)(const REAL *f, REAL tau0, REAL les_coeff) {
+    REAL rho = (REAL)0, jx = (REAL)0, jy = (REAL)0, jz = (REAL)0;
+    for (int i = 0; i < 19; ++i) {
+        rho = rho + f[i];
+        jx = jx + (REAL)ORACLE_E[i][0] * f[i];
+        jy = jy + (REAL)ORACLE_E[i][1] * f[i];
+        jz = jz + (REAL)ORACLE_E[i][2] * f[i];
+    }
+    return FN(oracle_les_inv_tau)(f, rho, jx, jy, jz, tau0, les_coeff);
+}
+
 /* Body force (new, forcing mode 1): f_i += 3 w_i (e_i . F), F = rho*g.
  * Adds momentum rho*g and no mass per step. */
 static void FN(oracle_body_force)(REAL *f, REAL rho, REAL gx, REAL gy, REAL gz) {

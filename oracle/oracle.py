@@ -71,6 +71,8 @@ def load():
             real = C.c_double if sfx == "_d" else C.c_float
             getattr(lib, "oracle_equilibrium" + sfx).argtypes = [real, real, real, real, P]
             getattr(lib, "oracle_equilibrium" + sfx).restype = None
+            getattr(lib, "oracle_les_rate" + sfx).argtypes = [P, real, real]
+            getattr(lib, "oracle_les_rate" + sfx).restype = real
             for n in ("oracle_alpha", "oracle_beta", "oracle_init", "oracle_store_dd_region",
                       "oracle_set_dd_region", "oracle_copy_dd_box", "oracle_moments"):
                 getattr(lib, n + sfx).restype = None
@@ -96,6 +98,16 @@ def equilibrium(rho, jx, jy, jz, dtype=np.float64):
     sfx = "_d" if np.dtype(dtype) == np.float64 else "_f"
     getattr(lib, "oracle_equilibrium" + sfx)(rho, jx, jy, jz, _p(out))
     return out
+
+
+def les_inv_tau(f, tau0, coeff, dtype=np.float64):
+    """1/tau_eff of the oracle's Smagorinsky rate for one cell's 19
+    populations, rho and j summed from them; `coeff` is the constant
+    18*sqrt(2)*Cs^2 (les_coeff)."""
+    lib = load()
+    fa = np.ascontiguousarray(np.asarray(f, dtype=dtype))
+    sfx = "_d" if np.dtype(dtype) == np.float64 else "_f"
+    return float(getattr(lib, "oracle_les_rate" + sfx)(_p(fa), tau0, coeff))
 
 
 def les_coeff(cs):

@@ -9,7 +9,8 @@ fails one of them:
   and an independent numpy evaluation of w_i(rho + 3 e.u + 4.5(e.u)^2 - 1.5u^2);
 * alpha/beta: 2k steps of the AA pattern against 2k steps of an independent
   numpy two-lattice collide-and-stream (textbook push scheme), per flag type;
-* the Smagorinsky rate against its closed form evaluated in numpy float64;
+* the Smagorinsky rate against its defining relation nu = nu0 + Cs^2 |S|
+  (strain rate from the tensor-route stress; planted errors fail it);
 * conservation of mass and momentum on a fully periodic (all-FLUID) lattice;
 * the seeded generator against the published splitmix64 first output;
 * region store/set and the norm filter against plain numpy slicing;
@@ -53,6 +54,91 @@ def test_equilibrium_moments_closed_form(seed):
     np.testing.assert_allclose(f, feq_numpy(np.array(rho), u.reshape(3)), rtol=0, atol=1e-15)
 
 
+# ------------------------------------------------------------ Smagorinsky rate
+def smagorinsky_tau(pi_neq, rho, tau0, cs, iters=60):
+    """tau_eff solved from the model's defining relation, without its closed
+    form: nu = nu0 + Cs^2 |S| with nu = (tau - 1/2)/3, the strain rate of the
+    non-equilibrium stress S = -3 Pi_neq / (2 rho tau) and |S| = sqrt(2 S:S)
+    (Hou, Sterling, Chen & Doolen 1996).  Newton's method on
+    F(tau) = (tau - 1/2)/3 - (tau0 - 1/2)/3 - Cs^2 |S(tau)|, vectorised over
+    the trailing axes."""
+    tau = np.full(np.shape(rho), float(tau0), dtype=np.float64)
+    for _ in range(iters):
+        S = -3.0 * pi_neq / (2.0 * rho * tau)
+        s_norm = np.sqrt(2.0 * np.einsum("ab...,ab...->...", S, S))
+        F = (tau - 0.5) / 3.0 - (tau0 - 0.5) / 3.0 - cs * cs * s_norm
+        dF = 1.0 / 3.0 + cs * cs * s_norm / tau  # d|S|/dtau = -|S|/tau
+        tau = tau - F / dF
+    return tau
+
+
+def pi_neq_tensor(f):
+    """Non-equilibrium stress of the incompressible equilibrium by the tensor
+    route: sum_i e_ia e_ib f_i - rho/3 delta_ab - j_a j_b."""
+    rho = f.sum(0)
+    j = np.einsum("ia,i...->a...", E, f)
+    eye = np.eye(3).reshape((3, 3) + (1,) * (f.ndim - 1))
+    return np.einsum("ia,ib,i...->ab...", E, E, f) - rho / 3 * eye - np.einsum("a...,b...->ab...", j, j), rho
+
+
+def _noneq_cells(seed, n, amp):
+    """Populations near equilibrium with a random non-equilibrium part."""
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 0.05 * rng.standard_normal(n)
+    u = 0.05 * rng.standard_normal((3, n))
+    f = feq_numpy(rho, u * rho)
+    return f + amp * W[:, None] * rng.standard_normal((19, n))
+
+
+@pytest.mark.parametrize("tau0", [0.5005, 0.52, 0.6])
+@pytest.mark.parametrize("cs", [0.1, 0.17])
+def test_smagorinsky_rate_satisfies_defining_relation(tau0, cs):
+    """The oracle's 1/tau_eff (the same closed form the device evaluates)
+    satisfies (tau_eff - 1/2)/3 = (tau0 - 1/2)/3 + Cs^2 |S| to 1e-12, with
+    S = -3 Pi_neq / (2 rho tau_eff) from the tensor route and |S| =
+    sqrt(2 S:S)."""
+    f = _noneq_cells(int(tau0 * 1e4) + int(cs * 100), 64, 2e-2)
+    pi, rho = pi_neq_tensor(f)
+    coeff = orc.les_coeff(cs)
+    for c in range(f.shape[1]):
+        tau = 1.0 / orc.les_inv_tau(f[:, c], tau0, coeff)
+        S = -3.0 * pi[:, :, c] / (2.0 * rho[c] * tau)
+        s_norm = np.sqrt(2.0 * np.sum(S * S))
+        assert cs * cs * s_norm > 1e-5, "the eddy viscosity must be significant for the check to bite"
+        assert abs((tau - 0.5) / 3.0 - ((tau0 - 0.5) / 3.0 + cs * cs * s_norm)) <= 1e-12
+    # the Newton solution of the same relation agrees with the oracle's closed form
+    np.testing.assert_allclose(smagorinsky_tau(pi, rho, tau0, cs),
+                               [1.0 / orc.les_inv_tau(f[:, c], tau0, coeff) for c in range(f.shape[1])],
+                               rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("planted", ["coeff_18", "coeff_sqrt2", "minus_root", "rho0_divisor"])
+def test_smagorinsky_relation_catches_planted_errors(planted):
+    """Plausible mistakes in the rate violate the defining relation: the
+    constant 18*sqrt(2) as 18 or as sqrt(2)*9, the other root of the
+    quadratic, and rho0 = 1 in place of rho."""
+    tau0, cs = 0.5005, 0.17
+    f = _noneq_cells(11, 16, 2e-2)
+    pi, rho = pi_neq_tensor(f)
+    worst = 0.0
+    for c in range(f.shape[1]):
+        coeff = orc.les_coeff(cs)
+        if planted == "coeff_18":
+            tau = 1.0 / orc.les_inv_tau(f[:, c], tau0, 18.0 * cs * cs)
+        elif planted == "coeff_sqrt2":
+            tau = 1.0 / orc.les_inv_tau(f[:, c], tau0, 9.0 * np.sqrt(2.0) * cs * cs)
+        elif planted == "minus_root":
+            q = np.sqrt(np.sum(pi[:, :, c] ** 2))
+            tau = 0.5 * (tau0 - np.sqrt(tau0 * tau0 + coeff * q / rho[c]))
+        else:
+            q = np.sqrt(np.sum(pi[:, :, c] ** 2))
+            tau = 0.5 * (tau0 + np.sqrt(tau0 * tau0 + coeff * q))
+        S = -3.0 * pi[:, :, c] / (2.0 * rho[c] * tau)
+        s_norm = np.sqrt(2.0 * np.sum(S * S))
+        worst = max(worst, abs((tau - 0.5) / 3.0 - ((tau0 - 0.5) / 3.0 + cs * cs * s_norm)))
+    assert worst > 1e-9, f"planted error {planted} was not caught"
+
+
 # ------------------------------------------------------------ textbook model
 def textbook_collide(f, flags, parity_alpha, inv_tau, lid, tau0=0.0, cs=0.0, body=None, ref_g=None):
     """Collision of every cell, float64, written from the textbook:
@@ -68,8 +154,7 @@ def textbook_collide(f, flags, parity_alpha, inv_tau, lid, tau0=0.0, cs=0.0, bod
         if cs > 0:
             pi = np.einsum("ia,ib,ic->abc", E, E, f[:, fl]) - rho[fl] / 3 * np.eye(3)[:, :, None] \
                 - np.einsum("ac,bc->abc", j[:, fl], j[:, fl])
-            q = np.sqrt(np.einsum("abc,abc->c", pi, pi))
-            tau = 0.5 * (tau0 + np.sqrt(tau0 * tau0 + 18 * np.sqrt(2) * cs * cs * q / rho[fl]))
+            tau = smagorinsky_tau(pi, rho[fl], tau0, cs)
             w = 1.0 / tau
         else:
             w = inv_tau
