@@ -69,3 +69,64 @@ def test_group_rejects_empty():
     lib = _capi.load_library()
     h = C.c_void_p()
     assert lib.lbm_group_create(None, 0, C.byref(h)) == _capi.LBM_ERR_INVALID_ARGUMENT
+
+
+def _variant_accepted_by(lib, variant, wg=256):
+    """(solver, group, server) acceptance of one launch shape, decided before
+    any device call: a shape that passes the checks fails later at the device
+    (no GPU here) with a different code."""
+    cfg = _capi.SolverConfig()
+    lib.lbm_solver_config_default(C.byref(cfg))
+    cfg.dims[0] = cfg.dims[1] = cfg.dims[2] = 16
+    cfg.kernel_variant = variant
+    cfg.work_group_size = wg
+    nbytes = C.c_size_t()
+    solver_ok = lib.lbm_solver_required_bytes(C.byref(cfg), C.byref(nbytes)) == _capi.LBM_OK
+    g = C.c_void_p()
+    group_ok = lib.lbm_group_create_empty(0, _capi.STORAGE_F32, wg, variant, C.byref(g)) != \
+        _capi.LBM_ERR_INVALID_ARGUMENT
+    scfg = _capi.ServerConfig()
+    lib.lbm_server_config_default(C.byref(scfg))
+    scfg.kernel_variant = variant
+    scfg.work_group_size = wg
+    scfg.memory_budget = 1 << 24
+    srv = C.c_void_p()
+    server_ok = lib.lbm_server_create(C.byref(scfg), C.byref(srv)) != _capi.LBM_ERR_INVALID_ARGUMENT
+    for h, destroy in ((g, lib.lbm_group_destroy), (srv, lib.lbm_server_destroy)):
+        if h.value:
+            destroy(h)
+    return solver_ok, group_ok, server_ok
+
+
+def test_solver_group_and_server_accept_the_same_launch_shapes():
+    lib = _capi.load_library()
+    accepted = []
+    for v in range(-1, 33):
+        s, g, srv = _variant_accepted_by(lib, v)
+        assert s == g == srv, f"variant {v}: solver {s}, group {g}, server {srv}"
+        if s:
+            accepted.append(v)
+    assert accepted == [0, 1, 2, 3, 4, 5, 6, 7, 8, 11, 12, 15, 16, 20]
+    for wg in (32, 64, 100, 128, 192, 256, 320):
+        s, g, srv = _variant_accepted_by(lib, 11, wg)
+        assert s == g == srv == (wg in (64, 128, 192, 256))
+
+
+def test_required_bytes_do_not_depend_on_launch_shape():
+    """A case sized once (as the server does) fits a solver of any accepted
+    work-group size and variant: the tile table is sized for the most tiles."""
+    lib = _capi.load_library()
+    sizes = set()
+    for dims in ((32, 32, 32), (17, 23, 29), (96, 48, 48)):
+        sizes.clear()
+        for wg in (64, 128, 192, 256):
+            for v in (0, 3, 8, 11, 16):
+                cfg = _capi.SolverConfig()
+                lib.lbm_solver_config_default(C.byref(cfg))
+                for a in range(3):
+                    cfg.dims[a] = dims[a]
+                cfg.work_group_size, cfg.kernel_variant = wg, v
+                n = C.c_size_t()
+                assert lib.lbm_solver_required_bytes(C.byref(cfg), C.byref(n)) == _capi.LBM_OK
+                sizes.add(n.value)
+        assert len(sizes) == 1, (dims, sizes)

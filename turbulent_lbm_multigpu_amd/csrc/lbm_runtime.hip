@@ -17,6 +17,7 @@
 
 #include "../../include/lbm_capi.h"
 #include "exchange_plan.hpp"
+#include "launch_shape.hpp"
 #include "lbm_device.hpp"
 
 using namespace lbm;
@@ -594,14 +595,17 @@ extern "C" void lbm_solver_config_default(lbm_solver_config *c) {
 
 // Device-memory layout of one solver: every buffer 256-byte aligned, in the
 // order distributions, flags, velocity, density, launch descriptor, tile
-// table, staging.  Staging holds one byte per cell (obstacle masks) or the 19
-// populations of the largest face, whichever is more; larger transfers go
-// region by region.
+// table, staging.  The tile table is sized for the most tiles any launch
+// shape needs (work-groups of 64, one cell per lane), so the size does not
+// depend on work_group_size or kernel_variant.  Staging holds one byte per
+// cell (obstacle masks) or the 19 populations of the largest face, whichever
+// is more; larger transfers go region by region.
 struct SolverLayout {
     size_t dd = 0, flags = 0, vel = 0, dens = 0, desc = 0, tile = 0, staging = 0, staging_bytes = 0, total = 0;
 };
 
-static SolverLayout solver_layout(const lbm_solver_config &c, long long cells, long long padded, int ntiles) {
+static SolverLayout solver_layout(const lbm_solver_config &c, long long cells, long long padded) {
+    const long long max_tiles = (cells + 63) / 64;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t es = elem_size(c.storage), cs = compute_size(c.storage);
     SolverLayout L;
@@ -621,7 +625,7 @@ static SolverLayout solver_layout(const lbm_solver_config &c, long long cells, l
     L.desc = off;
     off += al(sizeof(DomDev));
     L.tile = off;
-    off += al(sizeof(int) * (size_t)ntiles);
+    off += al(sizeof(int) * (size_t)max_tiles);
     L.staging = off;
     {
         const size_t X = c.dims[0], Y = c.dims[1], Z = c.dims[2];
@@ -651,17 +655,15 @@ static int validate_config(const lbm_solver_config &c) {
                       "the distributions of one (sub)domain must stay below 4 GiB: split the domain into subdomains");
     if (c.forcing != 0 && c.forcing != 1) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "unknown forcing");
     const int wg = c.work_group_size == 0 ? 256 : c.work_group_size;
-    if (wg < 64 || wg > 256 || wg % 64 != 0)
+    if (!lbm_work_group_supported(wg))
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "work_group_size must be 64, 128, 192 or 256");
     if (!(c.inv_tau > 0.0) || !std::isfinite(c.inv_tau))
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "inv_tau must be positive and finite");
     if (c.smagorinsky_cs < 0.0 || (c.smagorinsky_cs > 0.0 && !(c.tau > 0.5)))
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "LES needs smagorinsky_cs >= 0 and tau > 0.5");
-    const int v = c.kernel_variant & ~4;
-    if (c.kernel_variant < 0 || (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 11 && v != 16) || c.lds_per_block < 0 ||
-        c.lds_per_block > 65536)
+    if (!lbm_variant_supported(c.kernel_variant) || c.lds_per_block < 0 || c.lds_per_block > 65536)
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
-                      "kernel_variant must be one of 0, 1, 2, 3, 8, 16 (optionally | 4); lds_per_block in [0, 65536]");
+                      "kernel_variant must be one of 0, 1, 2, 3, 8, 11, 16 (optionally | 4); lds_per_block in [0, 65536]");
     return LBM_OK;
 }
 
@@ -683,7 +685,7 @@ static int create_solver(const lbm_solver_config *cfg, void *memory, size_t memo
     long long cells, padded;
     int wg, ntiles;
     solver_geometry(c, cells, padded, wg, ntiles);
-    const SolverLayout L = solver_layout(c, cells, padded, ntiles);
+    const SolverLayout L = solver_layout(c, cells, padded);
     if (memory && (memory_bytes < L.total || ((uintptr_t)memory & 255) != 0))
         return record(nullptr, LBM_ERR_CAPACITY,
                       "device memory block of " + std::to_string(memory_bytes) + " bytes (256-byte aligned) is smaller than the " +
@@ -756,7 +758,7 @@ extern "C" int lbm_solver_required_bytes(const lbm_solver_config *cfg, size_t *b
     long long cells, padded;
     int wg, ntiles;
     solver_geometry(*cfg, cells, padded, wg, ntiles);
-    *bytes = solver_layout(*cfg, cells, padded, ntiles).total;
+    *bytes = solver_layout(*cfg, cells, padded).total;
     return LBM_OK;
 }
 
@@ -1452,9 +1454,8 @@ extern "C" int lbm_group_create_empty(int device, int storage, int work_group_si
                                       lbm_group **out) {
     if (!out) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    const int v = kernel_variant & ~4;
-    if (storage < 0 || storage > 3 || work_group_size < 64 || work_group_size > 256 || work_group_size % 64 != 0 ||
-        kernel_variant < 0 || (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 11 && v != 16))
+    if (storage < 0 || storage > 3 || !lbm_work_group_supported(work_group_size) ||
+        !lbm_variant_supported(kernel_variant))
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad storage or launch shape");
     lbm_group *g = new lbm_group();
     g->storage = storage;
@@ -1887,9 +1888,7 @@ extern "C" int lbm_group_step(lbm_group *g, int steps) {
 
 extern "C" int lbm_group_set_launch(lbm_group *g, int work_group_size, int kernel_variant) {
     if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
-    const int v = kernel_variant & ~4;
-    if (work_group_size < 64 || work_group_size > 256 || work_group_size % 64 != 0 || kernel_variant < 0 ||
-        (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 11 && v != 16))
+    if (!lbm_work_group_supported(work_group_size) || !lbm_variant_supported(kernel_variant))
         return gfail(g, LBM_ERR_INVALID_ARGUMENT, "unsupported launch shape");
     GCHK(hipStreamSynchronize(g->stream));
     g->wg = work_group_size;

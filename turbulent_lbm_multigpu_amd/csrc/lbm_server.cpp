@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/lbm_capi.h"
+#include "launch_shape.hpp"
 
 namespace {
 
@@ -227,13 +228,23 @@ int case_setup(lbm_server *s, Case *c) {
     return LBM_OK;
 }
 
-// The case leaves the server as failed: its solver and arena block go, its
-// error is the server's last one.
-void case_fail(lbm_server *s, Case *c) {
-    if (c->solver) lbm_solver_destroy(c->solver);
-    c->solver = nullptr;
+// Destroys the case's solver and returns its arena block.  A solver that
+// cannot be destroyed (still a group member) keeps its block allocated, so
+// admission can never hand memory that a group still steps to another case.
+bool case_release(lbm_server *s, Case *c) {
+    if (c->solver) {
+        if (lbm_solver_destroy(c->solver) != LBM_OK) return false;
+        c->solver = nullptr;
+    }
     s->arena.release(c->offset, c->bytes);
     s->stats.bytes_in_use -= align256(c->bytes);
+    return true;
+}
+
+// The case leaves the server as failed: its solver and arena block go (see
+// case_release), its error is the server's last one.
+void case_fail(lbm_server *s, Case *c) {
+    (void)case_release(s, c);
     c->state = LBM_CASE_FAILED;
     c->error = s->err;
     c->done_s = s->now();
@@ -242,8 +253,11 @@ void case_fail(lbm_server *s, Case *c) {
 }
 
 // Joins the case's storage pool; on failure the group is left as it was (the
-// member and the comms naming it are removed again).
-int pool_join(lbm_server *s, Case *c) {
+// member and the comms naming it are removed again).  If that removal fails
+// too, the case stays a member (in step with the group's member indices, its
+// block allocated), *stuck is set and the server reports the error.
+int pool_join(lbm_server *s, Case *c, bool *stuck) {
+    *stuck = false;
     const int slot = storage_slot(c->c.storage);
     int rc = pool_open(s, slot);
     if (rc) return rc;
@@ -268,7 +282,13 @@ int pool_join(lbm_server *s, Case *c) {
     }
     if (rc) {
         const int code = s->fail_lib(rc, step, lbm_group_error(p.g));
-        (void)lbm_group_remove_member(p.g, idx);
+        const std::string why = s->err;
+        if (lbm_group_remove_member(p.g, idx) != LBM_OK) {
+            p.members.push_back(c);
+            c->state = LBM_CASE_RUNNING;
+            s->err = why + "; leaving the group again failed: " + lbm_group_error(p.g);
+            *stuck = true;
+        }
         return code;
     }
     p.members.push_back(c);
@@ -308,10 +328,7 @@ int case_readback(lbm_server *s, Case *c) {
             for (size_t i = 0; i < n; ++i)
                 c->j[(size_t)a * c->cells + at + i] = f64 ? (float)jb[a * n + i] : jf[a * n + i];
     }
-    lbm_solver_destroy(c->solver);
-    c->solver = nullptr;
-    s->arena.release(c->offset, c->bytes);
-    s->stats.bytes_in_use -= align256(c->bytes);
+    if (!case_release(s, c)) return s->fail(LBM_ERR_STATE, "readback: the case's solver cannot be released");
     c->state = LBM_CASE_DONE;
     c->done_s = s->now();
     s->done_order.push_back(c->id);
@@ -384,6 +401,7 @@ extern "C" int lbm_server_case_bytes(const lbm_case *c, size_t *bytes) {
     if (!c || !bytes) return LBM_ERR_INVALID_ARGUMENT;
     if (c->kind != LBM_CASE_CAVITY && c->kind != LBM_CASE_CHANNEL) return LBM_ERR_INVALID_ARGUMENT;
     if (storage_slot(c->storage) < 0) return LBM_ERR_INVALID_ARGUMENT;
+    // the solver's size does not depend on the launch shape the server uses
     lbm_solver_config cfg;
     case_solver_config(*c, 0, 256, 0, cfg);
     return lbm_solver_required_bytes(&cfg, bytes);
@@ -392,10 +410,9 @@ extern "C" int lbm_server_case_bytes(const lbm_case *c, size_t *bytes) {
 extern "C" int lbm_server_create(const lbm_server_config *cfg, lbm_server **out) {
     if (!cfg || !out) return LBM_ERR_INVALID_ARGUMENT;
     *out = nullptr;
-    const int v = cfg->kernel_variant & ~4;
     if (cfg->memory_budget < ((size_t)1 << 20) || cfg->max_active < 0 || !(cfg->chunk_ms > 0.0) ||
-        cfg->max_chunk_steps < 2 || cfg->work_group_size < 64 || cfg->work_group_size > 256 ||
-        cfg->work_group_size % 64 != 0 || cfg->kernel_variant < 0 || (v != 0 && v != 1 && v != 2 && v != 3 && v != 8 && v != 16))
+        cfg->max_chunk_steps < 2 || !lbm_work_group_supported(cfg->work_group_size) ||
+        !lbm_variant_supported(cfg->kernel_variant))
         return LBM_ERR_INVALID_ARGUMENT;
     std::unique_ptr<lbm_server> s(new lbm_server());
     s->cfg = *cfg;
@@ -503,12 +520,20 @@ extern "C" int lbm_server_progress(lbm_server *s, int *completed) {
     // fails on its own, as in admission
     std::vector<Case *> ready;
     ready.swap(s->ready);
-    for (Case *c : ready) {
+    for (size_t i = 0; i < ready.size(); ++i) {
+        Case *c = ready[i];
         if (c->c.steps == 0) {
             s->retiring.push_back(c);
             continue;
         }
-        if (pool_join(s, c) != LBM_OK) case_fail(s, c);
+        bool stuck = false;
+        const int jr = pool_join(s, c, &stuck);
+        if (jr == LBM_OK) continue;
+        if (stuck) {  // the group no longer matches the pool: stop here, keep the rest ready
+            s->ready.insert(s->ready.end(), ready.begin() + (long)i + 1, ready.end());
+            return jr;
+        }
+        case_fail(s, c);
     }
     // 4. the next chunk, one graph-replayed step sequence per storage
     for (Pool &p : s->pools) {
