@@ -119,13 +119,39 @@ def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_b
     return grp
 
 
-def cpu_baseline(budget_s=12.0):
-    """The C oracle (OpenMP, all host threads) on the same workload: build the
-    2x2x2 decomposition, run steps until the budget is spent."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(repeats=5, warmup=20, window=100, cavity_steps=1000):
+    """The C oracle built for this host (-O3 -march=native, OpenMP over every
+    thread the process may use) on the same AA-pattern algorithm:
+
+    * the 256^3 LES cavity as 2x2x2 subdomains: `warmup` steps, then
+      `repeats` windows of `window` steps, median MLUPS (the headline's CPU
+      counterpart);
+    * the 64^3 BGK cavity (tau 0.6, lid 0.05): `cavity_steps` steps in full,
+      median of `repeats` runs.
+
+    The timing build allows FMA contraction; the parity build of the tests
+    does not.  Both compile the same source."""
+    import statistics
+    import tempfile
+
     import numpy as np
 
+    from oracle import build_oracle
     from oracle import oracle as orc
     from turbulent_lbm_multigpu_amd import decomposition as dec
+
+    lib_path = build_oracle.build_timing(tempfile.mkdtemp(prefix="lbm_oracle_timing_"))
+    threads = int(orc.load(lib_path).oracle_num_threads())
 
     w = WORKLOAD
     mono = dec.monolithic_size(w["domain"], w["nums"])
@@ -133,7 +159,7 @@ def cpu_baseline(budget_s=12.0):
     for sd in dec.decompose(w["domain"], w["nums"]):
         bc = [sd.bc[0][0], sd.bc[0][1], sd.bc[1][0], sd.bc[1][1], sd.bc[2][0], sd.bc[2][1]]
         d = orc.OracleDomain(sd.domain.size, bc, inv_tau=1.0 / w["tau"], tau=w["tau"], smagorinsky_cs=w["cs"],
-                             lid_velocity=w["lid"], store_velocity=False, store_density=False)
+                             lid_velocity=w["lid"], store_velocity=False, store_density=False, lib_path=lib_path)
         d.reset(amplitude=w["amp"], seed=w["seed"], key_origin=dec.key_origin(sd), key_dims=mono)
         if sd.has_lid:
             o, sz = dec.lid_region(sd.domain.size)
@@ -141,23 +167,38 @@ def cpu_baseline(budget_s=12.0):
         subs.append({"domain": d, "comms": [(c.dst_id, c.send_origin, c.recv_origin, c.send_size,
                                              c.comm_direction) for c in sd.comms]})
     multi = orc.OracleMulti(subs)
-    multi.run(2)  # warm-up
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        multi.run(2)
-        steps += 2
-        el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 200:
-            break
+    multi.run(warmup)
     cells = float(np.prod(w["domain"]))
+    les = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        multi.run(window)
+        les.append(cells * window / (time.perf_counter() - t0) / 1e6)
+    del multi, subs
+
+    cav = []
+    for _ in range(repeats):
+        dims = (64, 64, 64)
+        d = orc.OracleDomain(dims, (1,) * 6, inv_tau=1.0 / 0.6, tau=0.6, lid_velocity=0.05,
+                             store_velocity=False, store_density=False, lib_path=lib_path)
+        o, sz = dec.lid_region(dims)
+        d.set_flags(o, sz, orc.FLAG_INJECTION)
+        t0 = time.perf_counter()
+        d.run(cavity_steps)
+        cav.append(64 ** 3 * cavity_steps / (time.perf_counter() - t0) / 1e6)
     return {
-        "value": round(cells * steps / el / 1e6, 2),
+        "value": round(statistics.median(les), 2),
         "unit": "MLUPS",
-        "cores": int(orc.load().oracle_num_threads()),
+        "cores": threads,
         "kind": "port",
-        "sample": f"C oracle (OpenMP), 256^3 LES as 2x2x2 subdomains, {steps} steps after 2 warm-up steps, "
-                  f"{el:.1f} s",
+        "sample": f"C oracle built -O3 -march=native with OpenMP ({threads} threads, {_cpu_model()}, "
+                  f"{os.cpu_count()} CPUs visible): 256^3 LES as 2x2x2 subdomains, {warmup} warm-up steps then "
+                  f"median of {repeats} windows of {window} steps",
+        "cpu_model": _cpu_model(),
+        "windows_mlups": [round(v, 2) for v in les],
+        "cavity_64_bgk_mlups": round(statistics.median(cav), 2),
+        "cavity_64_bgk_sample": f"64^3 BGK cavity, tau 0.6, lid 0.05, {cavity_steps} steps in full, median of "
+                                f"{repeats} runs",
     }
 
 
@@ -189,7 +230,7 @@ def main():
     ap.add_argument("--fused-x", type=int, default=1, choices=[0, 1],
                     help="exchange the x faces inside the step kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--cpu-repeats", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
@@ -245,7 +286,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_budget)
+        cpu = cpu_baseline(args.cpu_repeats)
         print(f"[bench] CPU baseline {cpu['value']} MLUPS on {cpu['cores']} threads", file=sys.stderr, flush=True)
 
     out = {

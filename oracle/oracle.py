@@ -33,7 +33,7 @@ E = np.array(
      [0, 0, 1], [0, 0, -1], [0, 0, 0]], dtype=np.int64)
 OPP = np.array([i ^ 1 if i < 18 else 18 for i in range(19)])
 
-_lib = None
+_libs = {}
 _lock = threading.Lock()
 
 
@@ -49,14 +49,16 @@ class _ParamsD(C.Structure):
                 ("forcing", C.c_int), ("store_velocity", C.c_int), ("store_density", C.c_int)]
 
 
-def load():
-    global _lib
+def load(path=None):
+    """The oracle library: the parity build by default, or another build of
+    the same source (the timing build of the CPU baseline) at `path`."""
+    path = path or LIB
     with _lock:
-        if _lib is not None:
-            return _lib
-        if not os.path.exists(LIB):
-            raise ImportError(f"{LIB} not found: run python oracle/build_oracle.py")
-        lib = C.CDLL(LIB)
+        if path in _libs:
+            return _libs[path]
+        if not os.path.exists(path):
+            raise ImportError(f"{path} not found: run python oracle/build_oracle.py")
+        lib = C.CDLL(path)
         P = C.c_void_p
         I = C.c_int
         for sfx, params in (("_f", _ParamsF), ("_d", _ParamsD)):
@@ -79,7 +81,7 @@ def load():
         lib.oracle_unit.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]
         lib.oracle_unit.restype = C.c_double
         lib.oracle_num_threads.restype = C.c_int
-        _lib = lib
+        _libs[path] = lib
         return lib
 
 
@@ -119,8 +121,8 @@ class OracleDomain:
 
     def __init__(self, dims, bc=(1, 1, 1, 1, 1, 1), *, inv_tau, tau=0.0, smagorinsky_cs=0.0,
                  lid_velocity=0.0, gravitation=(0.0, 0.0, 0.0), forcing=0, dtype=np.float32,
-                 store_velocity=True, store_density=True):
-        self.lib = load()
+                 store_velocity=True, store_density=True, lib_path=None):
+        self.lib = load(lib_path)
         self.dims = tuple(int(d) for d in dims)
         self.C = int(np.prod(self.dims))
         self.dtype = np.dtype(dtype)

@@ -39,6 +39,7 @@ class CController {
     std::vector<CComm<T> *> _comm_container;
     lbm::CDeviceContext _ctx;
     lbm_group *_group = nullptr;  // owned by the manager when several subdomains share the device
+    bool _drives_group = false;   // this controller's computeNextStep steps _group
 
     // src/CController.hpp:83-226 (device selection and solver construction)
     int initLBMSolver() {
@@ -70,7 +71,11 @@ public:
         for (auto *c : _comm_container) delete c;
     }
 
-    void attachGroup(lbm_group *g) { _group = g; }
+    // `drives`: this controller steps the group (one per group)
+    void attachGroup(lbm_group *g, bool drives = true) {
+        _group = g;
+        _drives_group = drives;
+    }
     lbm_group *group() const { return _group; }
 
     // after alpha: neighbours' inner layers -> this group's ghost layers
@@ -83,9 +88,15 @@ public:
     }
 
     // src/CController.hpp:385-391.  A controller attached to a group steps the
-    // whole group (all subdomains advance together on the device).
+    // whole group: every subdomain advances together in one fused launch.  The
+    // reference runs one controller per rank; here one controller per group
+    // drives the loop (CManager::run), so a caller must not loop over the
+    // controllers of one group, or the group would advance once per
+    // controller.  Only the group's driving controller steps it; any other
+    // controller of the group throws.
     void computeNextStep() {
         if (_group) {
+            if (!_drives_group) throw "computeNextStep: this controller's group is stepped by its driving controller";
             if (lbm_group_step(_group, 1) != LBM_OK) throw "lbm_group_step failed";
             cLbmPtr->syncStepCounter();
             return;

@@ -151,7 +151,8 @@ public:
             throw "CManager: <exchange-mode> must be 0 or 1";
         if (lbm_group_set_fused_x(_group, cfg->fused_x) != LBM_OK)
             throw "CManager: <fused-x> must be 0, 1 or 2";
-        for (auto *ctl : _controllers) ctl->attachGroup(_group);
+        // the first controller drives the group; stepping through any other throws
+        for (size_t i = 0; i < _controllers.size(); ++i) _controllers[i]->attachGroup(_group, i == 0);
     }
 
     // src/CManager.hpp:206-211 + the loop of src/CController.hpp:396-522 for all subdomains
