@@ -61,31 +61,66 @@ def batching_leg(log):
     cases = batch.make_cavities(512, 20260101)
     b, _ = batch.run_batched(cases, 1000, autotune_pairs=2)
     s, _ = batch.run_sequential(cases, 1000)
-    log(f"[bench] batching: {b['cases_per_s']} cases/s batched, {s['cases_per_s']} one at a time")
+    log(f"[bench] batching: {b['cases_per_s']} cases/s batched ({b['graph_launches']} graph launch for "
+        f"{b['steps']} steps), {s['cases_per_s']} one at a time")
     return {"workload": "512 LES cavities, edges U{16..96}, lid U[0.02,0.1], tau lognormal(0.52,0.01) >= 0.5002, "
                         "Cs 0.17, FP32, 1000 steps, seed 20260101",
-            "cases_per_s": b["cases_per_s"], "mlups": b["mlups"], "wall_s": b["wall_s"], "tune_s": b["tune_s"],
+            "cases_per_s": b["cases_per_s"], "mlups": b["mlups"], "mlups_after_tuning": b["mlups_after_tuning"],
+            "wall_s": b["wall_s"], "tune_s": b["tune_s"], "run_s": b["run_s"],
+            "graph_launches": b["graph_launches"], "eager_half_steps": b["eager_half_steps"],
             "launch": b["launch"], "one_at_a_time_cases_per_s": s["cases_per_s"],
             "speedup": round(b["cases_per_s"] / s["cases_per_s"], 3)}
 
 
-def serving_leg(log, requests=24, rate=4.0):
+def serving_leg(log, requests=100, rates=(6.0, 12.0, 24.0), seed=20260101):
+    """Config 5: one seeded stream of cavity/channel requests, each with its
+    own storage (FP32 or FP16-shifted).  At saturation every leg serves the
+    same requests, so the storage and batching contributions show apart; at
+    three Poisson rates the plain FP32 path and the mixed stream with
+    batching give cases/s and p50/p99 latency; a sample of the served cases is
+    re-served in FP64 for the error of the served fields."""
     from turbulent_lbm_multigpu_amd import serving
 
-    # both paths tune their launch shape on their first chunk (the same tuner)
-    sat = serving.compare(serving.request_stream(requests, 0.0, 20260101), 200e9, autotune=True)
-    log(f"[bench] serving at saturation: {json.dumps(sat)}")
-    lat = serving.compare(serving.request_stream(requests, rate, 20260101), 200e9, autotune=True)
-    log(f"[bench] serving at {rate}/s: {json.dumps(lat)}")
-    b = lat["fp16_batched"]
-    return {"workload": f"{requests} cavity/channel requests, cubic edge log-uniform 32-192, steps U{{500..5000}}, "
-                        "200 GB budget, seed 20260101, launch shape autotuned per storage pool",
+    # every pool tunes its launch shape on its first chunk (the same tuner)
+    sat = serving.evaluate(serving.request_stream(requests, 0.0, seed), 200e9, autotune=True, log=log)
+    speed = sat["cases_per_s_vs_fp32_one_at_a_time"]
+    log(f"[bench] serving at saturation, cases/s vs the plain FP32 path: {json.dumps(speed)}")
+    at_rate = {}
+    for rate in rates:
+        r = serving.evaluate(serving.request_stream(requests, rate, seed), 200e9,
+                             legs=("fp32_one_at_a_time", "mixed_batched"), autotune=True, log=log)
+        at_rate[str(rate)] = {leg: {k: r[leg][k] for k in ("cases_per_s", "p50_latency_s", "p99_latency_s",
+                                                          "device_busy")}
+                              for leg in ("fp32_one_at_a_time", "mixed_batched")}
+        log(f"[bench] serving at {rate}/s: {json.dumps(at_rate[str(rate)])}")
+    err = serving.served_error_vs_fp64(serving.request_stream(requests, 0.0, seed), 200e9, log=log)
+    mix = serving.request_stream(requests, 0.0, seed)
+    return {"workload": f"{requests} cavity/channel requests (seed {seed}), cubic edge log-uniform 32-192, steps "
+                        "U{500..5000}, storage FP32 or FP16-shifted per request, 200 GB budget, launch shape "
+                        "autotuned per storage pool",
+            "fp16_share_of_requests": round(sum(r.storage == "f16" for r in mix) / len(mix), 3),
             "cases_per_s": sat["fp16_batched"]["cases_per_s"],
             "fp32_one_at_a_time_cases_per_s": sat["fp32_one_at_a_time"]["cases_per_s"],
-            "speedup_cases_per_s": sat["speedup_cases_per_s"],
-            "rate_per_s": rate, "p50_latency_s": b["p50_latency_s"], "p99_latency_s": b["p99_latency_s"],
-            "fp32_one_at_a_time_p50_latency_s": lat["fp32_one_at_a_time"]["p50_latency_s"],
-            "fp32_one_at_a_time_p99_latency_s": lat["fp32_one_at_a_time"]["p99_latency_s"]}
+            "speedup_cases_per_s": speed["fp16_batched"],
+            "saturation": {k: sat[k] for k in serving.LEGS},
+            "cases_per_s_vs_fp32_one_at_a_time": speed,
+            "poisson": at_rate,
+            "served_error_vs_fp64": err}
+
+
+def precision_leg(log, dims=(384, 192, 192), steps=10000):
+    """Config 3: the LES channel stored as FP64, FP32, FP16(f - w) and
+    BF16(f - w), errors of rho, u and kinetic energy against the FP64 run."""
+    from turbulent_lbm_multigpu_amd import precision
+
+    res = precision.run(dims, steps, ("f64", "f32", "f16", "bf16"))
+    log(f"[bench] precision: {json.dumps(res)}")
+    keep = ("mlups", "kernel_variant", "rho_max_abs_err", "u_max_abs_err_over_U", "u_rms_err_over_U", "ke_rel_err",
+            "mass_rel_err", "obstacle_fill", "U_ref")
+    return {"workload": f"LES channel {dims[0]}x{dims[1]}x{dims[2]}, periodic x, body force g_x 1e-6, tau 0.51, "
+                        f"Cs 0.17, seeded spheres r 3-8 filling 5%, {steps} steps from rest; errors over fluid cells "
+                        "against the FP64 run (FP64 mass and kinetic-energy reductions on the GPU)",
+            "storages": {k: {q: v[q] for q in keep if q in v} for k, v in res.items()}}
 
 
 def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_block=0, fused_x=True):
@@ -227,12 +262,16 @@ def main():
     ap.add_argument("--kernel-variant", type=int, default=3)
     ap.add_argument("--lds-per-block", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--graph-pairs", type=int, default=1000,
+                    help="most beta+alpha pairs one graph launch replays")
     ap.add_argument("--fused-x", type=int, default=1, choices=[0, 1],
                     help="exchange the x faces inside the step kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=100)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -247,8 +286,14 @@ def main():
           file=sys.stderr, flush=True)
     grp = build_group(args.exchange_mode, not args.no_graph, args.wg, args.kernel_variant,
                       args.lds_per_block, bool(args.fused_x))
+    grp.set_graph_pairs(args.graph_pairs)
+    # the plain-copy reference of the roofline, measured before the step loop
+    # (untimed; it also brings the device out of idle)
+    copy_gbs = copy_probe(lib)
     grp.step(args.warmup)
     grp.wait()
+    # capture the graphs of the timed steps beforehand (recording runs nothing)
+    grp.prepare_steps(args.steps)
     lib.lbm_device_synchronize(0)
     t0 = time.perf_counter()
     grp.step(args.steps)
@@ -270,7 +315,6 @@ def main():
     print(f"[bench] step kernel {kern_ms * 1e3:.1f} us/launch (beta {tm['beta_kernel_ms'] * 1e3:.1f}, "
           f"alpha {tm['alpha_kernel_ms'] * 1e3:.1f}), exchange {tm['exchange_ms'] * 1e3:.1f} us/launch",
           file=sys.stderr, flush=True)
-    copy_gbs = copy_probe(lib)
     mass, ke = grp.solvers[0].mass_energy()
 
     def log(m):
@@ -281,7 +325,8 @@ def main():
     for s in grp.solvers:
         s.close()
     batching = None if args.no_batch else batching_leg(log)
-    serving_res = None if args.no_serving else serving_leg(log)
+    precision_res = None if args.no_precision else precision_leg(log)
+    serving_res = None if args.no_serving else serving_leg(log, requests=args.serving_requests)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -309,6 +354,7 @@ def main():
             "init": "f_eq(1, du), du ~ U(+-1e-4), seed 20260101",
             "exchange_mode": args.exchange_mode, "work_group": args.wg, "graph": not args.no_graph,
             "kernel_variant": args.kernel_variant, "fused_x": bool(args.fused_x),
+            "graph_pairs": args.graph_pairs,
         },
         "roofline": {
             "bound": "hbm",
@@ -317,6 +363,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": round(traffic) if traffic else None,
+            # the whole step (kernel, exchange and launch gaps): MLUPS x 152 B against 8 TB/s
+            "frac_end_to_end": round(mlups * 1e6 * BYTES_PER_CELL / (HBM_PEAK_GBS * 1e9), 4),
         },
         "cpu_baseline": cpu,
         "step_kernel_us": round(kern_ms * 1e3, 2),
@@ -326,6 +374,7 @@ def main():
         "exchange_kernel_us": round(tm["exchange_ms"] * 1e3, 2),
         "subdomain0_mass_ke": [mass, ke],
         "batching": batching,
+        "precision": precision_res,
         "serving": serving_res,
     }
     print(json.dumps(out), flush=True)

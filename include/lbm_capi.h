@@ -206,8 +206,21 @@ int lbm_group_set_exchange_mode(lbm_group *g, int mode);
  * kernels lose more to the handoff tiles than the copy launch costs). */
 #define LBM_FUSED_X_AUTO 2
 int lbm_group_set_fused_x(lbm_group *g, int mode);
-/* replay the step pair as a hipGraph (1) or launch eagerly (0) */
+/* replay the step pairs as hipGraphs (1) or launch eagerly (0) */
 int lbm_group_set_graph(lbm_group *g, int enable);
+/* most beta+alpha step pairs one graph launch replays (1..4096, default 64):
+ * lbm_group_step(g, n) replays its pairs in ceil(pairs / this) graph launches.
+ * A graph is captured once per launch shape and pair count; the last few are
+ * kept (new; the reference enqueues every kernel, src/CLbmSolver.hpp:664-676) */
+int lbm_group_set_graph_pairs(lbm_group *g, int pairs);
+/* captures the graphs a following lbm_group_step(g, steps) from the current
+ * step counter replays, without running anything, so that call only launches
+ * (new) */
+int lbm_group_prepare_steps(lbm_group *g, int steps);
+/* how lbm_group_step has launched work so far: graph launches, half-steps
+ * launched without a graph, and graphs captured (new) */
+int lbm_group_get_launch_counts(const lbm_group *g, long long *graph_launches, long long *eager_half_steps,
+                                long long *graphs_captured);
 /* computeNextStep() for all members, `steps` times (src/CController.hpp:385-391) */
 int lbm_group_step(lbm_group *g, int steps);
 /* one exchange on its own: syncAlpha (after_beta = 0) or syncBeta

@@ -114,14 +114,16 @@ def test_periodic_channel_member_freeze():
 
 
 def test_batched_cavities_equal_sequential():
-    """The batching workload at reduced size: batched (autotuned) and
-    one-at-a-time runs give bit-identical moments."""
+    """The batching workload at reduced size: batched (autotuned, restarted,
+    then every step in one graph launch) and one-at-a-time runs give
+    bit-identical moments."""
     say("case: 12 cavities batched vs sequential, 60 steps")
     cases = batch.make_cavities(12, seed=5, edge=(16, 30))
     rb, fb = batch.run_batched(cases, 60, autotune_pairs=1, keep=True)
     rs, fs = batch.run_sequential(cases, 60, keep=True)
     say(f"  batched {rb['launch']}, sequential {rs['cases_per_s']} cases/s")
-    assert rb["autotune"] is not None  # the tuner's 12 shapes ran inside the 60 steps
+    assert rb["autotune"] is not None  # the tuner's 12 shapes ran, then the cases restarted
+    assert rb["graph_launches"] == 1 and rb["eager_half_steps"] == 0   # 60 steps in one graph launch
     for (r1, j1), (r2, j2) in zip(fb, fs):
         np.testing.assert_array_equal(r1, r2)
         np.testing.assert_array_equal(j1, j2)
@@ -137,6 +139,7 @@ def test_full_size_batch_equals_sequential():
     rs, fs = batch.run_sequential(cases, 60, keep=True)
     say(f"  batched {rb['launch']}, {rb['cells']} cells")
     assert rb["autotune"] is not None
+    assert rb["graph_launches"] == 1 and rb["eager_half_steps"] == 0
     for (r1, j1), (r2, j2) in zip(fb, fs):
         np.testing.assert_array_equal(r1, r2)
         np.testing.assert_array_equal(j1, j2)

@@ -56,3 +56,166 @@ def test_cli_flags_and_vtk_output(tmp_path):
     head = (tmp_path / "output" / "vtk" / "OUTPUT.1.2.vtk").read_text().splitlines()[:8]
     assert head[0].startswith("# vtk DataFile") and "STRUCTURED_GRID" in head[4]
     assert head[5].split() == ["DIMENSIONS", "13", "17", "17"]
+
+
+# --------------------------------------------------------------------------
+# The caller surface anchored to the oracle: conf.xml -> CConfiguration ->
+# CLbmSkeleton -> CManager/CController -> CLbmSolver -> kernels, then the
+# reference's outputs (debug checksum, VTK files, benchmark ini) compared
+# with the CPU oracle run of the same case.
+
+def _oracle_checksum(vel, flags):
+    """getVelocityChecksum (src/CLbmSolver.hpp:1103-1123): float sum over
+    FLUID cells of vx + vy + vz, in cell order, times 1000.0f."""
+    import numpy as np
+
+    fl = flags == 2
+    per = (vel[0][fl] + vel[1][fl]) + vel[2][fl]
+    return np.float32(np.cumsum(per.astype(np.float32), dtype=np.float32)[-1]) * np.float32(1000.0), \
+        float(np.abs(vel[:, fl]).sum()) * 1000.0
+
+
+def _conf(path, *, size, nums, loops, physics, vtk=0, benchmark=0, lattice=""):
+    visc, g, cav = physics
+    open(path, "w").write(f"""<?xml version="1.0"?>
+<lbm-configuration>
+  <physics>
+    <viscosity>{visc}</viscosity>
+    <gravitation><x>{g[0]}</x><y>{g[1]}</y><z>{g[2]}</z></gravitation>
+    <cavity-velocity><x>{cav}</x><y>0</y><z>0</z><w>1</w></cavity-velocity>
+  </physics>
+  <grid>
+    <domain-size><x>{size[0]}</x><y>{size[1]}</y><z>{size[2]}</z></domain-size>
+    <subdomain-num><x>{nums[0]}</x><y>{nums[1]}</y><z>{nums[2]}</z></subdomain-num>
+    <domian-length><x>0.1</x><y>0.1</y><z>0.1</z></domian-length>
+  </grid>
+  <simulation>
+    <loops>{loops}</loops>
+    <timestep>-1.0</timestep>
+    <visualization><VTK>{vtk}</VTK></visualization>
+    <validate>0</validate>
+    <benchmark>{benchmark}</benchmark>
+  </simulation>
+  <device>
+    <kernel-count>256</kernel-count>
+    <device-number>0</device-number>
+  </device>{lattice}
+</lbm-configuration>
+""")
+
+
+def test_config1_through_physical_surface_matches_oracle(tmp_path):
+    """Config 1 (64^3 BGK cavity, tau 0.6, lid 0.05, 1000 steps) expressed in
+    the reference's physical units (g_y = -9.81 turns on the time-step limiter:
+    dt = sqrt(1e-4 dx / 9.81), nu = 6.44825e-4 gives tau = 0.6, U = 396.18
+    gives lid = 0.05).  The driver's debug checksum equals the oracle's run
+    with the float32 parametrization to 1e-5 of sum |v|; the benchmark ini
+    holds the reference's keys and formulas (src/CController.hpp:445-489)."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    from tests.test_host_cpp import skeleton_float32
+    from turbulent_lbm_multigpu_amd import decomposition as dec
+
+    n, steps = 64, 1000
+    nu, U = 6.44825e-4, 396.18
+    conf = str(tmp_path / "conf_config1.xml")
+    _conf(conf, size=(n, n, n), nums=(1, 1, 1), loops=steps, physics=(nu, (0, -9.81, 0), U), benchmark=1)
+    r = run("-c", conf, "-v", cwd=tmp_path, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "limiting timestep" in r.stdout
+    got = float(re.search(r"Checksum: ([-0-9.]+)", r.stdout).group(1))
+
+    dx, dt, tau, inv_tau, g, lid = skeleton_float32(n, 0.1, -9.81, nu, U)
+    assert abs(float(tau) - 0.6) < 1e-5 and abs(float(lid) - 0.05) < 1e-5
+    o = orc.OracleDomain((n, n, n), (1,) * 6, inv_tau=float(inv_tau), tau=float(tau), lid_velocity=float(lid),
+                         gravitation=tuple(float(v) for v in g))
+    origin, size = dec.lid_region((n, n, n))
+    o.set_flags(origin, size, orc.FLAG_INJECTION)
+    o.run(steps)
+    want, scale = _oracle_checksum(o.vel, o.flags)
+    print(f"checksum: driver {got:.8f}, oracle {float(want):.8f}, scale {scale:.3e}", flush=True)
+    assert scale > 1.0
+    assert abs(got - float(want)) <= 1e-5 * scale
+
+    # the report lines and the benchmark ini (floats per cell: 19*2 + 1 + 3 in debug mode)
+    secs = float(re.search(r"Seconds: ([0-9.eE+-]+)", r.stdout).group(1))
+    assert f"Cube: [{n}, {n}, {n}]" in r.stdout
+    ini = (tmp_path / "output" / "benchmark" / "benchmark_1.ini").read_text()
+    kv = dict((k.strip(), float(v)) for k, v in (ln.split(":") for ln in ini.splitlines() if ":" in ln))
+    assert set(kv) == {"CUBE_X", "CUBE_Y", "CUBE_Z", "SECONDS", "FPS", "MLUPS", "BANDWIDTH"}
+    assert kv["CUBE_X"] == kv["CUBE_Y"] == kv["CUBE_Z"] == n
+    assert kv["SECONDS"] == pytest.approx(secs, rel=1e-5)
+    assert kv["FPS"] == pytest.approx(steps / kv["SECONDS"], rel=1e-5)
+    assert kv["MLUPS"] == pytest.approx(kv["FPS"] * n ** 3 * 1e-6, rel=1e-5)
+    assert kv["BANDWIDTH"] == pytest.approx(kv["MLUPS"] * 42 * 4, rel=1e-5)
+    mlups = float(re.search(r"MLUPS: ([0-9.eE+-]+)", r.stdout).group(1))
+    assert mlups == pytest.approx(kv["MLUPS"], rel=1e-5)
+
+
+def _read_vtk(path):
+    """Legacy-ASCII STRUCTURED_GRID of CLbmVisualizationVTK: dims, points,
+    cell density, flag and velocity."""
+    import numpy as np
+
+    lines = open(path).read().splitlines()
+    i = next(k for k, ln in enumerate(lines) if ln.startswith("DIMENSIONS"))
+    dims = tuple(int(v) for v in lines[i].split()[1:4])
+    npts = int(lines[i + 1].split()[1])
+    pts = np.array([ln.split() for ln in lines[i + 3:i + 3 + npts]], dtype=np.float64)
+    k = next(k for k, ln in enumerate(lines) if ln.startswith("CELL_DATA"))
+    ncell = int(lines[k].split()[1])
+    dens = np.array(lines[k + 3:k + 3 + ncell], dtype=np.float64)
+    k2 = k + 3 + ncell
+    assert lines[k2 + 1].startswith("SCALARS flag INT")
+    flags = np.array(lines[k2 + 3:k2 + 3 + ncell], dtype=np.int64)
+    k3 = k2 + 3 + ncell
+    assert lines[k3 + 1].startswith("VECTORS velocity float")
+    vel = np.array([ln.split() for ln in lines[k3 + 2:k3 + 2 + ncell]], dtype=np.float64).T
+    return dims, pts, dens, flags, vel
+
+
+def test_decomposed_les_vtk_output_matches_oracle(tmp_path):
+    """A 2x1x1 decomposed LES cavity with the seeded perturbation, configured
+    by the <lattice> block: each subdomain's VTK file after 11 steps holds the
+    oracle decomposition's density output, flags and velocity output
+    (src/libvis/CLbmVisualizationVTK.hpp:27-99), to 1e-5 relative plus the
+    %f print resolution (5e-7)."""
+    import numpy as np
+
+    from tests.test_oracle import build_oracle_decomposition
+
+    size, nums, steps = (34, 26, 22), (2, 1, 1), 11
+    tau, lid, cs, amp, seed = 0.5005, 0.1, 0.17, 1e-4, 20260101
+    conf = str(tmp_path / "conf_les_vtk.xml")
+    lattice = f"""
+  <lattice>
+    <tau>{tau}</tau>
+    <lid-velocity>{lid}</lid-velocity>
+    <smagorinsky-cs>{cs}</smagorinsky-cs>
+    <exchange-mode>1</exchange-mode>
+    <perturbation><amplitude>{amp}</amplitude><seed>{seed}</seed></perturbation>
+  </lattice>"""
+    _conf(conf, size=size, nums=nums, loops=steps, physics=(0.001308, (0, 0, 0), 100), vtk=1, lattice=lattice)
+    r = run("-c", conf, cwd=tmp_path, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    # the C++ layer's float parameters: tau = (float)0.5005, inv_tau = 1.0f / tau
+    t32 = np.float32(tau)
+    kw = dict(inv_tau=float(np.float32(1.0) / t32), tau=float(t32), smagorinsky_cs=cs)
+    multi, _ = build_oracle_decomposition(size, nums, lid=float(np.float32(lid)), amp=amp, seed=seed, **kw)
+    multi.sync(after_alpha=True)   # CManager's initial ghost-layer copy (all 19 populations)
+    multi.run(steps)
+    dx = np.float32(0.1) / np.float32(size[0])
+    for uid, sub in enumerate(multi.subs):
+        d = sub["domain"]
+        dims, pts, dens, flags, vel = _read_vtk(tmp_path / "output" / "vtk" / f"OUTPUT.{uid}.{steps - 1}.vtk")
+        assert dims == tuple(v + 1 for v in d.dims)
+        ox = sub["sd"].domain.origin[0] * dx
+        assert pts[0] == pytest.approx([ox, 0, 0], abs=1e-6)
+        assert pts[-1] == pytest.approx([ox + d.dims[0] * dx, d.dims[1] * dx, d.dims[2] * dx], abs=1e-6)
+        np.testing.assert_array_equal(flags, d.flags)
+        assert (flags == 2).sum() > 0.5 * flags.size
+        np.testing.assert_allclose(dens, d.dens, rtol=1e-5, atol=5e-7)
+        np.testing.assert_allclose(vel, d.vel, rtol=1e-5, atol=5e-7)
+        assert np.abs(d.vel).max() > 1e-3 and np.abs(d.dens).max() > 0.5   # fields that can fail the check

@@ -60,7 +60,8 @@ EXPORTED_SYMBOLS = (
     "lbm_solver_set_lid_velocity", "lbm_solver_set_relaxation", "lbm_solver_store", "lbm_solver_set",
     "lbm_solver_velocity_checksum", "lbm_solver_mass_energy", "lbm_solver_moments", "lbm_copy_box",
     "lbm_solver_device_pointers", "lbm_group_create", "lbm_group_destroy", "lbm_group_error",
-    "lbm_group_add_comm", "lbm_group_set_exchange_mode", "lbm_group_set_graph", "lbm_group_step",
+    "lbm_group_add_comm", "lbm_group_set_exchange_mode", "lbm_group_set_graph", "lbm_group_set_graph_pairs",
+    "lbm_group_prepare_steps", "lbm_group_get_launch_counts", "lbm_group_step",
     "lbm_group_wait", "lbm_group_set_timing", "lbm_group_get_timing", "lbm_group_sync",
     "lbm_group_get_timing_detail", "lbm_solver_place_spheres", "lbm_group_set_launch",
     "lbm_group_get_launch", "lbm_group_autotune",
@@ -220,6 +221,10 @@ def load_library(path: str | None = None):
     sig("lbm_group_add_comm", I, P, I, I, IP, IP, IP, IP)
     sig("lbm_group_set_exchange_mode", I, P, I)
     sig("lbm_group_set_graph", I, P, I)
+    sig("lbm_group_set_graph_pairs", I, P, I)
+    sig("lbm_group_prepare_steps", I, P, I)
+    LL = C.POINTER(C.c_longlong)
+    sig("lbm_group_get_launch_counts", I, P, LL, LL, LL)
     sig("lbm_group_set_fused_x", I, P, I)
     sig("lbm_group_step", I, P, I)
     sig("lbm_group_wait", I, P)

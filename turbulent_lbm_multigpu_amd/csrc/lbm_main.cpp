@@ -171,9 +171,14 @@ int main(int argc, char **argv) {
         manager.initSimulation(0);
         manager.startSimulation();
         if (debug) {
+            // the debug checksum line of src/CController.hpp:490-497 (fixed, 8 decimals)
             auto *s = manager.getController()->getSolver();
+            const std::streamsize ss = std::cout.precision();
             std::cout.precision(8);
+            std::cout.setf(std::ios::fixed, std::ios::floatfield);
             std::cout << "Checksum: " << (s->getVelocityChecksum() * 1000.0f) << std::endl;
+            std::cout.precision(ss);
+            std::cout.unsetf(std::ios::floatfield);
         }
     } catch (const char *msg) {
         std::cerr << "error: " << msg << std::endl;

@@ -413,9 +413,19 @@ struct lbm_group {
     size_t chains_cap = 0;
     long long step_counter = 0;
     bool use_graph = true;
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    std::vector<long long> graph_shape;  // launch sizes and table addresses baked into the captured graph
+    int graph_pairs = 64;  // most beta+alpha pairs one graph launch replays
+    struct Graph {
+        int pairs;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        unsigned long long used;
+    };
+    std::vector<Graph> graphs;           // captured step sequences by pair count, all of graph_shape
+    unsigned long long graph_clock = 0;  // LRU order of graphs
+    std::vector<long long> graph_shape;  // launch sizes and table addresses baked into the captured graphs
+    long long graph_launches = 0;        // hipGraphLaunch calls of lbm_group_step
+    long long eager_half_steps = 0;      // half-steps lbm_group_step launched without a graph
+    long long graphs_captured = 0;       // graphs captured and instantiated
     bool timing = false;
     struct Ev {
         hipEvent_t a, b;
@@ -1268,10 +1278,11 @@ extern "C" int lbm_device_copy_bandwidth(int device, size_t bytes, int reps, int
 // ---- group -----------------------------------------------------------------
 
 static void group_drop_graph(lbm_group *g) {
-    if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
-    if (g->graph) (void)hipGraphDestroy(g->graph);
-    g->graph_exec = nullptr;
-    g->graph = nullptr;
+    for (auto &e : g->graphs) {
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipGraphDestroy(e.graph);
+    }
+    g->graphs.clear();
     g->graph_shape.clear();
 }
 
@@ -1839,6 +1850,86 @@ static void group_advance_counters(lbm_group *g, long long before, long long don
     g->step_counter = before + done;
 }
 
+// The graph replaying `pairs` beta+alpha pairs (with their exchanges):
+// captured once per launch shape and pair count, at most kMaxGraphs kept
+// (least recently used dropped first).  Capturing records launches without
+// running them.
+static constexpr size_t kMaxGraphs = 4;
+static int group_graph(lbm_group *g, int pairs, hipGraphExec_t *out) {
+    if (!g->graphs.empty() && g->graph_shape != group_shape(g)) group_drop_graph(g);
+    for (auto &e : g->graphs)
+        if (e.pairs == pairs) {
+            e.used = ++g->graph_clock;
+            *out = e.exec;
+            return LBM_OK;
+        }
+    if (g->graphs.size() >= kMaxGraphs) {
+        auto lru = std::min_element(g->graphs.begin(), g->graphs.end(),
+                                    [](const lbm_group::Graph &a, const lbm_group::Graph &b) { return a.used < b.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        (void)hipGraphDestroy(lru->graph);
+        g->graphs.erase(lru);
+    }
+    GCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = LBM_OK;
+    for (int p = 0; p < pairs && !rc; ++p) {
+        rc = group_launch_half(g, true, false);
+        if (!rc) rc = group_launch_half(g, false, false);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(g->stream, &graph);
+    if (rc || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc) return rc;
+        GCHK(ec);
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        GCHK(ei);
+    }
+    if (g->graphs.empty()) g->graph_shape = group_shape(g);
+    g->graphs.push_back({pairs, graph, exec, ++g->graph_clock});
+    g->graphs_captured += 1;
+    *out = exec;
+    return LBM_OK;
+}
+
+// Walks `steps` half-steps from the group's counter: graph replays of up to
+// graph_pairs pairs where a pair starts on a beta step, single eager launches
+// otherwise.  With run == false it only captures the graphs the walk needs.
+static int group_walk(lbm_group *g, int steps, bool run, int *done_out) {
+    long long h = g->step_counter;
+    int done = 0;
+    while (done < steps) {
+        const bool beta = (h & 1) == 0;
+        const int pairs = (steps - done) / 2;
+        if (g->use_graph && !g->timing && beta && pairs >= 1) {
+            const int P = std::min(pairs, g->graph_pairs);
+            hipGraphExec_t exec = nullptr;
+            int rc = group_graph(g, P, &exec);
+            if (rc) return rc;
+            if (run) {
+                GCHK(hipGraphLaunch(exec, g->stream));
+                g->graph_launches += 1;
+            }
+            h += 2 * P;
+            done += 2 * P;
+        } else {
+            if (run) {
+                int rc = group_launch_half(g, beta, g->timing);
+                if (rc) return rc;
+                g->eager_half_steps += 1;
+            }
+            h += 1;
+            done += 1;
+        }
+    }
+    *done_out = done;
+    return LBM_OK;
+}
+
 extern "C" int lbm_group_step(lbm_group *g, int steps) {
     if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
     if (steps < 0) return gfail(g, LBM_ERR_INVALID_ARGUMENT, "steps must be >= 0");
@@ -1851,38 +1942,36 @@ extern "C" int lbm_group_step(lbm_group *g, int steps) {
         group_advance_counters(g, before, steps);
         return LBM_OK;
     }
-    if (g->graph_exec && g->graph_shape != group_shape(g)) group_drop_graph(g);
-    long long h = before;
     int done = 0;
-    while (done < steps) {
-        const bool beta = (h & 1) == 0;
-        if (g->use_graph && !g->timing && beta && steps - done >= 2) {
-            if (!g->graph_exec) {
-                GCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-                int rc1 = group_launch_half(g, true, false);
-                int rc2 = rc1 ? rc1 : group_launch_half(g, false, false);
-                hipGraph_t graph = nullptr;
-                hipError_t ec = hipStreamEndCapture(g->stream, &graph);
-                if (rc2) {
-                    if (graph) (void)hipGraphDestroy(graph);
-                    return rc2;
-                }
-                GCHK(ec);
-                g->graph = graph;
-                GCHK(hipGraphInstantiate(&g->graph_exec, g->graph, nullptr, nullptr, 0));
-                g->graph_shape = group_shape(g);
-            }
-            GCHK(hipGraphLaunch(g->graph_exec, g->stream));
-            h += 2;
-            done += 2;
-        } else {
-            rc = group_launch_half(g, beta, g->timing);
-            if (rc) return rc;
-            h += 1;
-            done += 1;
-        }
-    }
+    rc = group_walk(g, steps, true, &done);
+    if (rc) return rc;
     group_advance_counters(g, before, done);
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_prepare_steps(lbm_group *g, int steps) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    if (steps < 0) return gfail(g, LBM_ERR_INVALID_ARGUMENT, "steps must be >= 0");
+    GCHK(hipSetDevice(g->device));
+    int rc = group_prepare(g);
+    if (rc || g->ntiles == 0) return rc;
+    int done = 0;
+    return group_walk(g, steps, false, &done);
+}
+
+extern "C" int lbm_group_get_launch_counts(const lbm_group *g, long long *graph_launches, long long *eager_half_steps,
+                                           long long *graphs_captured) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    if (graph_launches) *graph_launches = g->graph_launches;
+    if (eager_half_steps) *eager_half_steps = g->eager_half_steps;
+    if (graphs_captured) *graphs_captured = g->graphs_captured;
+    return LBM_OK;
+}
+
+extern "C" int lbm_group_set_graph_pairs(lbm_group *g, int pairs) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    if (pairs < 1 || pairs > 4096) return gfail(g, LBM_ERR_INVALID_ARGUMENT, "graph pairs must be in [1, 4096]");
+    g->graph_pairs = pairs;
     return LBM_OK;
 }
 

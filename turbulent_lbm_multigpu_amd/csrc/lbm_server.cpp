@@ -178,6 +178,10 @@ int pool_open(lbm_server *s, int slot) {
     rc = lbm_group_set_exchange_mode(p.g, 1);
     // periodic channel seams inside the step kernel for FP32/FP64, copy launch for FP16/BF16
     if (!rc) rc = lbm_group_set_fused_x(p.g, LBM_FUSED_X_AUTO);
+    // membership changes at most chunk boundaries, and a changed membership
+    // needs new graphs: one captured pair, replayed K/2 times, keeps the
+    // capture cost per chunk small
+    if (!rc) rc = lbm_group_set_graph_pairs(p.g, 1);
     if (rc) return s->fail_lib(rc, "exchange mode", lbm_group_error(p.g));
     void *st = nullptr;
     lbm_group_get_stream(p.g, &st);

@@ -295,6 +295,68 @@ def compare(requests: Sequence[Request], budget: float, *, chunk_ms: float = 25.
     return out
 
 
+# The legs of the serving evaluation: (name, storage override or None for
+# each request's own, max_active: 1 = one case at a time, 0 = continuous batching)
+LEGS = {
+    "fp32_one_at_a_time": ("f32", 1),    # the plain FP32 path
+    "fp16_one_at_a_time": ("f16", 1),    # shifted storage alone
+    "fp32_batched": ("f32", 0),          # continuous batching alone
+    "fp16_batched": ("f16", 0),          # both
+    "mixed_batched": (None, 0),          # each request's own storage (FP32 or FP16), batched
+}
+
+
+def evaluate(requests: Sequence[Request], budget: float, legs: Sequence[str] = tuple(LEGS), *,
+             chunk_ms: float = 25.0, work_group_size: int = 256, kernel_variant: int = 0, autotune: bool = False,
+             log=None) -> dict:
+    """Serve the same requests once per leg (see LEGS) and report each leg's
+    summary plus its cases/s relative to the plain FP32 path."""
+    out = {}
+    for name in legs:
+        storage, max_active = LEGS[name]
+        with Server(budget, max_active=max_active, chunk_ms=chunk_ms, work_group_size=work_group_size,
+                    kernel_variant=kernel_variant, autotune=autotune) as srv:
+            r = serve(srv, requests, storage=storage)
+        out[name] = r.summary()
+        if log:
+            log(f"  {name}: {json.dumps(out[name])}")
+    base = out.get("fp32_one_at_a_time", {}).get("cases_per_s")
+    if base:
+        out["cases_per_s_vs_fp32_one_at_a_time"] = {
+            k: round(v["cases_per_s"] / base, 3) for k, v in out.items() if isinstance(v, dict) and "cases_per_s" in v}
+    return out
+
+
+def served_error_vs_fp64(requests: Sequence[Request], budget: float, *, storages=("f16", "f32"), sample: int = 4,
+                         max_edge: int = 96, log=None) -> dict:
+    """Re-serve a sample of the requests (the first `sample` whose edge is at
+    most `max_edge`) in FP64 and in each of `storages`, and report the error
+    of the served fields against FP64 over the interior cells (boundary layers
+    excluded): largest |rho - rho64|, largest and rms |u - u64| / U with U the
+    largest |u64|.  The worst case of the sample is reported per storage."""
+    picked = [r for r in requests if r.edge <= max_edge][:sample]
+    fields = {}
+    for storage in ("f64",) + tuple(storages):
+        with Server(budget, max_active=0) as srv:
+            fields[storage] = serve(srv, picked, storage=storage, keep_results=True).results
+    out = {"sample": [{"kind": r.kind, "edge": r.edge, "steps": r.steps} for r in picked]}
+    for storage in storages:
+        worst = {"rho_max_abs_err": 0.0, "u_max_abs_err_over_U": 0.0, "u_rms_err_over_U": 0.0}
+        for k in range(len(picked)):
+            r64, j64 = (np.asarray(a, np.float64)[..., 1:-1, 1:-1, 1:-1] for a in fields["f64"][k])
+            r, j = (np.asarray(a, np.float64)[..., 1:-1, 1:-1, 1:-1] for a in fields[storage][k])
+            u64, u = j64 / r64, j / r
+            U = float(np.abs(u64).max())
+            worst["rho_max_abs_err"] = max(worst["rho_max_abs_err"], float(np.abs(r - r64).max()))
+            worst["u_max_abs_err_over_U"] = max(worst["u_max_abs_err_over_U"], float(np.abs(u - u64).max()) / U)
+            worst["u_rms_err_over_U"] = max(worst["u_rms_err_over_U"],
+                                            float(np.sqrt(np.mean((u - u64) ** 2))) / U)
+        out[storage] = {k: float(f"{v:.3g}") for k, v in worst.items()}
+        if log:
+            log(f"  served {storage} vs FP64: {out[storage]}")
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--requests", type=int, default=48)

@@ -315,6 +315,20 @@ class LbmGroup:
     def set_graph(self, enable: bool):
         self._check(self._lib.lbm_group_set_graph(self._h, int(bool(enable))))
 
+    def set_graph_pairs(self, pairs: int):
+        """Most beta+alpha pairs one graph launch replays."""
+        self._check(self._lib.lbm_group_set_graph_pairs(self._h, int(pairs)))
+
+    def prepare_steps(self, steps: int):
+        """Capture the graphs a following step(steps) replays, running nothing."""
+        self._check(self._lib.lbm_group_prepare_steps(self._h, int(steps)))
+
+    def launch_counts(self) -> dict:
+        """Graph launches and graph-less half-steps of step() so far, graphs captured."""
+        a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        self._check(self._lib.lbm_group_get_launch_counts(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"graph_launches": a.value, "eager_half_steps": b.value, "graphs_captured": c.value}
+
     def step(self, steps: int = 1):
         self._check(self._lib.lbm_group_step(self._h, int(steps)))
 
