@@ -124,7 +124,9 @@ def test_config1_through_physical_surface_matches_oracle(tmp_path):
     r = run("-c", conf, "-v", cwd=tmp_path, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "limiting timestep" in r.stdout
-    got = float(re.search(r"Checksum: ([-0-9.]+)", r.stdout).group(1))
+    m = re.search(r"Checksum: ([-0-9.]+)\s", r.stdout)
+    assert m, r.stdout[-600:]
+    got = float(m.group(1))
 
     dx, dt, tau, inv_tau, g, lid = skeleton_float32(n, 0.1, -9.81, nu, U)
     assert abs(float(tau) - 0.6) < 1e-5 and abs(float(lid) - 0.05) < 1e-5
@@ -145,12 +147,14 @@ def test_config1_through_physical_surface_matches_oracle(tmp_path):
     kv = dict((k.strip(), float(v)) for k, v in (ln.split(":") for ln in ini.splitlines() if ":" in ln))
     assert set(kv) == {"CUBE_X", "CUBE_Y", "CUBE_Z", "SECONDS", "FPS", "MLUPS", "BANDWIDTH"}
     assert kv["CUBE_X"] == kv["CUBE_Y"] == kv["CUBE_Z"] == n
+    # every value is printed with the stream's default 6 significant digits:
+    # two roundings of <= 5e-6 each separate a value from the one derived from it
     assert kv["SECONDS"] == pytest.approx(secs, rel=1e-5)
-    assert kv["FPS"] == pytest.approx(steps / kv["SECONDS"], rel=1e-5)
-    assert kv["MLUPS"] == pytest.approx(kv["FPS"] * n ** 3 * 1e-6, rel=1e-5)
-    assert kv["BANDWIDTH"] == pytest.approx(kv["MLUPS"] * 42 * 4, rel=1e-5)
+    assert kv["FPS"] == pytest.approx(steps / kv["SECONDS"], rel=1.1e-5)
+    assert kv["MLUPS"] == pytest.approx(kv["FPS"] * n ** 3 * 1e-6, rel=1.1e-5)
+    assert kv["BANDWIDTH"] == pytest.approx(kv["MLUPS"] * 42 * 4, rel=1.1e-5)
     mlups = float(re.search(r"MLUPS: ([0-9.eE+-]+)", r.stdout).group(1))
-    assert mlups == pytest.approx(kv["MLUPS"], rel=1e-5)
+    assert mlups == pytest.approx(kv["MLUPS"], rel=1.1e-5)
 
 
 def _read_vtk(path):
@@ -175,12 +179,21 @@ def _read_vtk(path):
     return dims, pts, dens, flags, vel
 
 
-def test_decomposed_les_vtk_output_matches_oracle(tmp_path):
+@pytest.mark.parametrize("exchange_mode", [0, 1])
+def test_decomposed_les_vtk_output_matches_oracle(tmp_path, exchange_mode):
     """A 2x1x1 decomposed LES cavity with the seeded perturbation, configured
     by the <lattice> block: each subdomain's VTK file after 11 steps holds the
     oracle decomposition's density output, flags and velocity output
     (src/libvis/CLbmVisualizationVTK.hpp:27-99), to 1e-5 relative plus the
-    %f print resolution (5e-7)."""
+    %f print resolution (5e-7).
+
+    The oracle exchanges all 19 populations after alpha, as the reference
+    does (exchange mode 0): then every cell agrees.  Exchange mode 1 copies
+    only the 5 populations a fluid cell reads from a ghost; the others stay
+    stale in the ghost layer.  The OBSTACLE cells of the outer x walls read
+    such slots through the linear wrap (their x - 1 or x + 1 neighbour is the
+    ghost cell of the adjacent row), so their density output -- never a fluid
+    value -- differs there; every other cell and every velocity agree."""
     import numpy as np
 
     from tests.test_oracle import build_oracle_decomposition
@@ -193,7 +206,7 @@ def test_decomposed_les_vtk_output_matches_oracle(tmp_path):
     <tau>{tau}</tau>
     <lid-velocity>{lid}</lid-velocity>
     <smagorinsky-cs>{cs}</smagorinsky-cs>
-    <exchange-mode>1</exchange-mode>
+    <exchange-mode>{exchange_mode}</exchange-mode>
     <perturbation><amplitude>{amp}</amplitude><seed>{seed}</seed></perturbation>
   </lattice>"""
     _conf(conf, size=size, nums=nums, loops=steps, physics=(0.001308, (0, 0, 0), 100), vtk=1, lattice=lattice)
@@ -216,6 +229,11 @@ def test_decomposed_les_vtk_output_matches_oracle(tmp_path):
         assert pts[-1] == pytest.approx([ox + d.dims[0] * dx, d.dims[1] * dx, d.dims[2] * dx], abs=1e-6)
         np.testing.assert_array_equal(flags, d.flags)
         assert (flags == 2).sum() > 0.5 * flags.size
-        np.testing.assert_allclose(dens, d.dens, rtol=1e-5, atol=5e-7)
+        keep = np.ones(flags.size, bool)
+        if exchange_mode == 1:
+            x = np.arange(flags.size) % d.dims[0]
+            wall = (flags == 1) & (((x == 0) & (uid == 0)) | ((x == d.dims[0] - 1) & (uid == len(multi.subs) - 1)))
+            keep = ~wall
+        np.testing.assert_allclose(dens[keep], d.dens[keep], rtol=1e-5, atol=5e-7)
         np.testing.assert_allclose(vel, d.vel, rtol=1e-5, atol=5e-7)
         assert np.abs(d.vel).max() > 1e-3 and np.abs(d.dens).max() > 0.5   # fields that can fail the check

@@ -147,3 +147,62 @@ def test_full_size_mix_alone_equals_in_the_mix():
         np.testing.assert_array_equal(mix.results[k][0], one.results[0][0])
         np.testing.assert_array_equal(mix.results[k][1], one.results[0][1])
     say(f"  {len(reqs)} cases identical")
+
+
+def _served(cases, **kw):
+    """Submit `cases` to one server, drive it to completion, return each
+    case's (rho, j) in submission order."""
+    with serving.Server(2e9, chunk_ms=0.5, max_chunk_steps=8, **kw) as srv:
+        ids = [srv.submit(c) for c in cases]
+        for _ in range(10000):
+            srv.progress()
+            if not srv.busy():
+                break
+        assert not srv.busy()
+        for i in ids:
+            assert srv.status(i)["state"] == _capi.CASE_DONE, srv.status(i)
+        return [srv.fetch(i) for i in ids]
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_served_channel_equals_plain_channel(storage):
+    """A served periodic channel (spheres, seeded perturbation, body force,
+    an odd step count) against the channel built outside the server
+    (precision.build_channel: periodic comms of the decomposition, ghost
+    planes synchronised once, FP32 seam inside the step kernel, FP16 seam by
+    the copy launch) and stepped by its own group: rho and j bit for bit."""
+    from turbulent_lbm_multigpu_amd import precision
+
+    dims, steps, seed, amp = (30, 22, 18), 37, 424242, 1e-4
+    say(f"case: served channel {dims} {storage}, {steps} steps vs build_channel")
+    case = serving.make_case("channel", dims, steps, storage, tau=0.51, cs=0.17, body_force=1e-5, fill=0.05,
+                             r_min=2.0, r_max=4.0, perturbation=amp, seed=seed)
+    (rho_s, j_s), = _served([case])
+    s, g, filled = precision.build_channel(dims, precision.STORAGES[storage], tau=0.51, cs=0.17, gx=1e-5,
+                                           sphere_seed=seed, fill=0.05, rmin=2.0, rmax=4.0, amp=amp, seed=seed)
+    assert filled > 0.02
+    g.step(steps)
+    g.wait()
+    rho, j = s.moments()
+    flags = s.store(_capi.FIELD_FLAGS)
+    g.close()
+    s.close()
+    assert (flags == _capi.FLAG_OBSTACLE).any()
+    assert np.abs(j).max() > 1e-5   # the flow the check can fail on
+    np.testing.assert_array_equal(rho_s, rho.astype(np.float32))
+    np.testing.assert_array_equal(j_s, j.astype(np.float32))
+
+
+@pytest.mark.parametrize("wg,variant", [(64, 0), (128, 8), (192, 3)])
+def test_server_small_work_groups(wg, variant):
+    """A server created with a work-group size below 256 serves its cases,
+    and their results equal the default launch shape's."""
+    say(f"case: server wg {wg} variant {variant}")
+    cases = lambda: [serving.make_case("cavity", (18, 18, 18), 15, "f32", lid=0.06, seed=5),
+                     serving.make_case("channel", (20, 16, 14), 12, "f16", fill=0.03, r_min=2.0, r_max=3.0,
+                                       perturbation=1e-4, seed=6)]
+    got = _served(cases(), work_group_size=wg, kernel_variant=variant)
+    want = _served(cases())
+    for (r0, j0), (r1, j1) in zip(got, want):
+        np.testing.assert_array_equal(r0, r1)
+        np.testing.assert_array_equal(j0, j1)

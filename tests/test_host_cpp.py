@@ -107,3 +107,27 @@ def test_parametrization_worked_example():
     assert abs(float(dt) - 1.26205e-4) < 1e-8
     assert abs(float(tau) - 0.6) < 1e-5
     assert abs(float(lid) - 0.05) < 1e-5
+
+
+@pytest.mark.parametrize("n,L,gy,nu,U", [
+    (64, 0.1, -9.81, 6.44825e-4, 396.18),     # config 1 in physical units: lid 0.05
+    (32, 0.1, -9.81, 0.001308, 100.0),
+])
+def test_solver_hands_the_lattice_lid_to_the_library(tmp_path, n, L, gy, nu, U):
+    """CLbmSolver passes the skeleton's lattice values (lid = U dt, not the
+    physical U it also keeps, src/CLbmSolver.hpp:567) to lbm_solver_create.
+    Linked against the built library; no device call succeeds or is needed."""
+    lib_dir = os.path.join(ROOT, "turbulent_lbm_multigpu_amd")
+    if not os.path.exists(os.path.join(lib_dir, "liblbm_hip.so")):
+        pytest.skip("liblbm_hip.so not built")
+    exe = str(tmp_path / "host_solver_selftest")
+    subprocess.run([shutil.which("g++") or "g++", "-O1", "-std=c++17", "-Wall", f"-I{CSRC}",
+                    f"-I{os.path.join(ROOT, 'include')}", os.path.join(ROOT, "tests", "cpp", "host_solver_selftest.cpp"),
+                    "-o", exe, f"-L{lib_dir}", "-llbm_hip", f"-Wl,-rpath,{lib_dir}"], check=True)
+    out = subprocess.run([exe, str(n), str(L), str(gy), str(nu), str(U)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    v = out.stdout.split()
+    got = dict(zip(v[0::2], v[1::2]))
+    dx, dt, tau, inv_tau, g, lid = skeleton_float32(n, L, gy, nu, U)
+    assert np.float32(got["lid"]) == np.float32(lid) and abs(float(lid)) < 0.2
+    assert np.float32(got["tau"]) == tau and np.float32(got["inv_tau"]) == inv_tau

@@ -112,7 +112,7 @@ public:
             CLbmSkeleton<T>::error.clear();
             this->tau = (T)_lattice.tau;
             this->inv_tau = (T)1.0 / this->tau;
-            this->drivenCavityVelocity[0] = (T)_lattice.lid_velocity;
+            CLbmSkeleton<T>::drivenCavityVelocity[0] = (T)_lattice.lid_velocity;
             for (int a = 0; a < 3; ++a) this->gravitation[a] = (T)_lattice.body_force[a];
         }
         if (CLbmSkeleton<T>::error()) error << CLbmSkeleton<T>::error.getString();
@@ -146,6 +146,14 @@ public:
         }
         if (error()) return;
         lbm_solver_config c;
+        solverConfig(c);
+        if (fail(lbm_solver_create(&c, &_h))) return;
+        simulation_step_counter = 0;
+    }
+
+    // the configuration reload() creates the device solver with: the
+    // skeleton's lattice values, the boundary flags and the launch shape
+    void solverConfig(lbm_solver_config &c) const {
         lbm_solver_config_default(&c);
         CVector<3, int> cells = this->domain_cells;
         for (int a = 0; a < 3; ++a) c.dims[a] = cells[a];
@@ -157,7 +165,9 @@ public:
         c.tau = (double)this->tau;
         c.smagorinsky_cs = _lattice.enabled ? _lattice.smagorinsky_cs : 0.0;
         for (int a = 0; a < 3; ++a) c.gravitation[a] = (double)this->gravitation[a];
-        c.lid_velocity = (double)this->drivenCavityVelocity[0];
+        // the lattice lid U dt of the parametrization, not the physical U of
+        // this class's own member (src/CLbmSolver.hpp:567)
+        c.lid_velocity = (double)CLbmSkeleton<T>::drivenCavityVelocity[0];
         c.store_velocity = store_velocity;
         c.store_density = store_density;
         int wg = (int)computation_kernel_count;
@@ -167,8 +177,6 @@ public:
         c.work_group_size = wg;
         c.kernel_variant = _lattice.kernel_variant;
         c.device = _ctx.device;
-        if (fail(lbm_solver_create(&c, &_h))) return;
-        simulation_step_counter = 0;
     }
 
     // src/CLbmSolver.hpp:619-635

@@ -94,6 +94,19 @@ def counter_means(path: str) -> Dict[str, float]:
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
+def counter_table(paths: Sequence[str], kernel: str = "") -> Dict[str, Dict[str, float]]:
+    """Mean value per dispatch of every counter, per kernel name (kernels whose
+    name contains `kernel`), from one or more rocprofv3 counter_collection
+    CSV files (one --pmc pass each)."""
+    acc = defaultdict(lambda: defaultdict(list))
+    for path in paths:
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                if kernel in row["Kernel_Name"]:
+                    acc[row["Kernel_Name"]][row["Counter_Name"]].append(float(row["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in acc.items()}
+
+
 def pmc_traffic(fetch_csv: str, write_csv: str, kernel: str, copy: str, copy_bytes: float,
                 fetch_scale: Optional[float] = None, write_scale: Optional[float] = None) -> dict:
     """HBM bytes per launch of every kernel whose name contains `kernel`.
@@ -142,8 +155,17 @@ def main(argv=None):
     b.add_argument("--copy-bytes", type=float, default=float(1 << 30))
     b.add_argument("--fetch-scale", type=float, default=None, help="use this FETCH_SIZE scale (no copy kernel)")
     b.add_argument("--write-scale", type=float, default=None, help="use this WRITE_SIZE scale (no copy kernel)")
+    c = sub.add_parser("counters")
+    c.add_argument("csv", nargs="+")
+    c.add_argument("--kernel", default="")
     args = ap.parse_args(argv)
-    if args.cmd == "ini":
+    if args.cmd == "counters":
+        tab = counter_table(args.csv, args.kernel)
+        names = sorted({n for cs in tab.values() for n in cs})
+        print("kernel\t" + "\t".join(names))
+        for k in sorted(tab):
+            print(k + "\t" + "\t".join(f"{tab[k].get(n, float('nan')):.6g}" for n in names))
+    elif args.cmd == "ini":
         ev = events_from_kernel_trace(args.trace, args.filter)
         write_event_ini(ev, args.out)
         print(f"{len(ev)} events, {len(overlapping(ev))} overlapping pairs", file=sys.stderr)
