@@ -156,6 +156,16 @@ int lbm_solver_velocity_checksum(lbm_solver *s, float *out);
 /* FP64 reductions over FLUID cells of rho and kinetic energy 1/2 |j|^2/rho,
  * moments taken from the distributions in the current step parity (new) */
 int lbm_solver_mass_energy(lbm_solver *s, double *mass, double *kinetic_energy);
+/* lbm_solver_mass_energy without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same kernels and summation order,
+ * hence the same bits.  Its partial sums live in a fixed per-device scratch
+ * array of 2 MiB, resolved once per device and held until the process ends:
+ * solvers of more than 33,554,432 cells (2^25) are rejected with
+ * LBM_ERR_CAPACITY.  Every call on one device shares that array, so calls
+ * that overlap on one device (from several host threads) race; calls from
+ * one thread do not, since each returns after its result is on the host. */
+int lbm_solver_mass_energy_noalloc(lbm_solver *s, double *mass, double *kinetic_energy);
 /* rho and j = sum f_i e_i of a region, from the distributions in the current
  * step parity, in the compute type: rho[size], j[3][size] (new) */
 int lbm_solver_moments(lbm_solver *s, const int origin[3], const int size[3], void *host_rho, void *host_j);

@@ -58,7 +58,8 @@ EXPORTED_SYMBOLS = (
     "lbm_solver_simulation_step_beta", "lbm_solver_run", "lbm_solver_wait",
     "lbm_solver_get_step_counter", "lbm_solver_set_step_counter", "lbm_solver_get_config",
     "lbm_solver_set_lid_velocity", "lbm_solver_set_relaxation", "lbm_solver_store", "lbm_solver_set",
-    "lbm_solver_velocity_checksum", "lbm_solver_mass_energy", "lbm_solver_moments", "lbm_copy_box",
+    "lbm_solver_velocity_checksum", "lbm_solver_mass_energy", "lbm_solver_mass_energy_noalloc",
+    "lbm_solver_moments", "lbm_copy_box",
     "lbm_solver_device_pointers", "lbm_group_create", "lbm_group_destroy", "lbm_group_error",
     "lbm_group_add_comm", "lbm_group_set_exchange_mode", "lbm_group_set_graph", "lbm_group_set_graph_pairs",
     "lbm_group_prepare_steps", "lbm_group_get_launch_counts", "lbm_group_step",
@@ -211,6 +212,7 @@ def load_library(path: str | None = None):
     sig("lbm_solver_set", I, P, I, IP, IP, P, IP)
     sig("lbm_solver_velocity_checksum", I, P, C.POINTER(C.c_float))
     sig("lbm_solver_mass_energy", I, P, C.POINTER(D), C.POINTER(D))
+    sig("lbm_solver_mass_energy_noalloc", I, P, C.POINTER(D), C.POINTER(D))
     sig("lbm_solver_moments", I, P, IP, IP, P, P)
     sig("lbm_copy_box", I, P, IP, P, IP, IP, IP)
     sig("lbm_solver_device_pointers", I, P, C.POINTER(P), C.POINTER(P), C.POINTER(C.c_longlong))

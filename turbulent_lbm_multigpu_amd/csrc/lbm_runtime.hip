@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -983,6 +984,33 @@ extern "C" int lbm_solver_velocity_checksum(lbm_solver *s, float *out) {
     return LBM_OK;
 }
 
+namespace {
+
+// The two reduction passes of the FP64 mass and kinetic energy into
+// `partials` (2 * (blocks + 1) doubles: the result pair, then one pair per
+// block; every element is written before it is read, so nothing needs
+// clearing), then the result pair to the host.
+int mass_energy_passes(lbm_solver *s, double *partials, int blocks, double *mass, double *ke) {
+    const int swapped = (s->step_counter % 2) == 0;
+    int rc = dispatch_storage(s->cfg.storage, [&](auto st) -> int {
+        constexpr int ST = decltype(st)::value;
+        mass_energy_kernel<ST><<<blocks, 256, 0, s->stream>>>(s->host_desc, swapped, partials + 2);
+        SCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (rc) return rc;
+    sum_pairs_kernel<<<1, 256, 0, s->stream>>>(partials + 2, blocks, partials);
+    SCHK(hipGetLastError());
+    double res[2];
+    SCHK(hipMemcpyAsync(res, partials, sizeof(res), hipMemcpyDeviceToHost, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    *mass = res[0];
+    *ke = res[1];
+    return LBM_OK;
+}
+
+}  // namespace
+
 extern "C" int lbm_solver_mass_energy(lbm_solver *s, double *mass, double *ke) {
     if (!s || !mass || !ke) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
     const int blocks = (int)((s->cells + 255) / 256);
@@ -993,22 +1021,59 @@ extern "C" int lbm_solver_mass_energy(lbm_solver *s, double *mass, double *ke) {
             return fail(s, LBM_ERR_OUT_OF_MEMORY, "reduction buffer allocation failed");
         s->partial_count = (size_t)blocks + 1;
     }
-    const int swapped = (s->step_counter % 2) == 0;
-    int rc = dispatch_storage(s->cfg.storage, [&](auto st) -> int {
-        constexpr int ST = decltype(st)::value;
-        mass_energy_kernel<ST><<<blocks, 256, 0, s->stream>>>(s->host_desc, swapped, s->d_partials + 2);
-        SCHK(hipGetLastError());
-        return LBM_OK;
-    });
-    if (rc) return rc;
-    sum_pairs_kernel<<<1, 256, 0, s->stream>>>(s->d_partials + 2, blocks, s->d_partials);
-    SCHK(hipGetLastError());
-    double res[2];
-    SCHK(hipMemcpyAsync(res, s->d_partials, sizeof(res), hipMemcpyDeviceToHost, s->stream));
-    SCHK(hipStreamSynchronize(s->stream));
-    *mass = res[0];
-    *ke = res[1];
+    return mass_energy_passes(s, s->d_partials, blocks, mass, ke);
+}
+
+// ---- allocation-free reduction (no reference counterpart) -------------------
+// Scratch of lbm_solver_mass_energy_noalloc: the result pair and one partial
+// pair per 256-cell block, for solvers of at most kScratchBlocks * 256 cells.
+// Each device holds its own copy until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress gives on that device.
+constexpr int kScratchBlocks = 1 << 17;
+static __device__ __align__(256) double scratch[2 * (kScratchBlocks + 1)];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of `scratch` on `device`, resolved once per device with
+// that device current.
+int scratch_address(lbm_solver *s, int device, double **out) {
+    if (device < 0 || device >= kScratchDevices)
+        return fail(s, LBM_ERR_DEVICE, "device ordinal " + std::to_string(device) + " is beyond the " +
+                                           std::to_string(kScratchDevices) + " devices the scratch table covers");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        int prev = 0;
+        SCHK(hipGetDevice(&prev));
+        SCHK(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        SCHK(hipSetDevice(prev));
+        if (e != hipSuccess || !p)
+            return fail(s, LBM_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = (double *)p;
+    }
+    *out = g_scratch_ptr[device];
     return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" int lbm_solver_mass_energy_noalloc(lbm_solver *s, double *mass, double *ke) {
+    if (!s || !mass || !ke) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    const long long blocks = (s->cells + 255) / 256;
+    if (blocks > kScratchBlocks)
+        return fail(s, LBM_ERR_CAPACITY,
+                    "lbm_solver_mass_energy_noalloc: " + std::to_string(s->cells) + " cells exceed the " +
+                        std::to_string((long long)kScratchBlocks * 256) +
+                        " cells its fixed scratch covers (use lbm_solver_mass_energy)");
+    double *ptr = nullptr;
+    const int rc = scratch_address(s, s->cfg.device, &ptr);
+    if (rc) return rc;
+    return mass_energy_passes(s, ptr, (int)blocks, mass, ke);
 }
 
 extern "C" int lbm_solver_moments(lbm_solver *s, const int origin[3], const int size[3], void *host_rho, void *host_j) {
