@@ -17,7 +17,7 @@ from turbulent_lbm_multigpu_amd import _capi  # noqa: E402
 from turbulent_lbm_multigpu_amd import precision  # noqa: E402
 from turbulent_lbm_multigpu_amd.solver import LbmSolver  # noqa: E402
 
-# Stated bounds for the reduced channel (96x48x48, 2000 steps): rho as an
+# Stated bounds for the channel after 2000 steps (96x48x48 and 384x192x192): rho as an
 # absolute error, u relative to the largest FP64 speed U over fluid cells, ke
 # relative.  FP32 storage runs the reference's arithmetic on f, where the
 # per-step body-force increment 3 w g ~ 1.7e-7 is only ~45 ulp of a population
@@ -55,9 +55,12 @@ def test_body_force_momentum_balance_fp64():
     s.close()
 
 
-def test_channel_storage_errors_vs_fp64():
-    say("case: channel 96x48x48 2000 steps, f64/f32/f16/bf16")
-    res = precision.run((96, 48, 48), 2000, ("f64", "f32", "f16", "bf16"))
+@pytest.mark.parametrize("dims", [(96, 48, 48), (384, 192, 192)])
+def test_channel_storage_errors_vs_fp64(dims):
+    """The reduced channel and the channel at its full size (384x192x192),
+    2000 steps each: the same stated bounds hold for both."""
+    say(f"case: channel {dims} 2000 steps, f64/f32/f16/bf16")
+    res = precision.run(dims, 2000, ("f64", "f32", "f16", "bf16"))
     say(f"  {res}")
     assert abs(res["f64"]["mass_end"] - res["f64"]["mass_start"]) / res["f64"]["mass_start"] < 1e-3
     assert 0.045 < res["f64"]["obstacle_fill"] < 0.06
