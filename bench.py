@@ -123,12 +123,13 @@ def precision_leg(log, dims=(384, 192, 192), steps=10000):
             "storages": {k: {q: v[q] for q in keep if q in v} for k, v in res.items()}}
 
 
-def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_block=0, fused_x=True):
+def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_block=0, fused_x=True,
+                colocated=False):
     import numpy as np
 
     from turbulent_lbm_multigpu_amd import _capi
     from turbulent_lbm_multigpu_amd import decomposition as dec
-    from turbulent_lbm_multigpu_amd.solver import LbmGroup, LbmSolver
+    from turbulent_lbm_multigpu_amd.solver import LbmGroup, LbmSolver, colocate
 
     w = WORKLOAD
     mono = dec.monolithic_size(w["domain"], w["nums"])
@@ -136,14 +137,16 @@ def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_b
     solvers = []
     for sd in subs:
         bc = [sd.bc[0][0], sd.bc[0][1], sd.bc[1][0], sd.bc[1][1], sd.bc[2][0], sd.bc[2][1]]
-        s = LbmSolver(sd.domain.size, bc, inv_tau=1.0 / w["tau"], tau=w["tau"], smagorinsky_cs=w["cs"],
-                      lid_velocity=w["lid"], work_group_size=wg,
-                      kernel_variant=kernel_variant, lds_per_block=lds_per_block)
+        solvers.append(LbmSolver(sd.domain.size, bc, inv_tau=1.0 / w["tau"], tau=w["tau"], smagorinsky_cs=w["cs"],
+                                 lid_velocity=w["lid"], work_group_size=wg,
+                                 kernel_variant=kernel_variant, lds_per_block=lds_per_block))
+    if colocated:
+        colocate(solvers)   # the eight subdomains side by side in one allocation
+    for sd, s in zip(subs, solvers):
         s.init_perturbed(w["amp"], w["seed"], 0, dec.key_origin(sd), mono)
         if sd.has_lid:
             o, sz = dec.lid_region(sd.domain.size)
             s.set(_capi.FIELD_FLAGS, np.full(sz[::-1], _capi.FLAG_VELOCITY_INJECTION, np.int32), o, sz)
-        solvers.append(s)
     grp = LbmGroup(solvers)
     for sd in subs:
         for c in sd.comms:
@@ -266,6 +269,8 @@ def main():
                     help="most beta+alpha pairs one graph launch replays")
     ap.add_argument("--fused-x", type=int, default=1, choices=[0, 1],
                     help="exchange the x faces inside the step kernel")
+    ap.add_argument("--colocate", action="store_true",
+                    help="move the eight subdomains into one shared allocation (default: allocations of their own)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true")
@@ -285,7 +290,7 @@ def main():
     print(f"[bench] building 256^3 LES 2x2x2 (exchange_mode={args.exchange_mode}, wg={args.wg})",
           file=sys.stderr, flush=True)
     grp = build_group(args.exchange_mode, not args.no_graph, args.wg, args.kernel_variant,
-                      args.lds_per_block, bool(args.fused_x))
+                      args.lds_per_block, bool(args.fused_x), args.colocate)
     grp.set_graph_pairs(args.graph_pairs)
     # the plain-copy reference of the roofline, measured before the step loop
     # (untimed; it also brings the device out of idle)
@@ -354,7 +359,7 @@ def main():
             "init": "f_eq(1, du), du ~ U(+-1e-4), seed 20260101",
             "exchange_mode": args.exchange_mode, "work_group": args.wg, "graph": not args.no_graph,
             "kernel_variant": args.kernel_variant, "fused_x": bool(args.fused_x),
-            "graph_pairs": args.graph_pairs,
+            "graph_pairs": args.graph_pairs, "colocated": args.colocate,
         },
         "roofline": {
             "bound": "hbm",

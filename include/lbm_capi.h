@@ -120,6 +120,15 @@ int lbm_solver_required_bytes(const lbm_solver_config *cfg, size_t *bytes);
  * than the staging area fail with LBM_ERR_CAPACITY (new) */
 int lbm_solver_create_in(const lbm_solver_config *cfg, void *device_memory, size_t bytes, void *stream,
                          lbm_solver **out);
+/* moves the device buffers of n solvers (created by lbm_solver_create, not yet
+ * in a group) into one device allocation, side by side, contents unchanged;
+ * each solver is still destroyed on its own and the allocation goes with the
+ * last of them.  Where buffers land in HBM changes the step time by a few
+ * percent either way depending on the machine, so this is a placement
+ * choice, not a default (new; the reference allocates per buffer and per
+ * rank, src/CLbmSolver.hpp:381-403).  Needs the new block while the old
+ * buffers are still held. */
+int lbm_solver_colocate(lbm_solver *const *solvers, int n);
 /* CError text of this solver (src/lib/CError.hpp:42-80) */
 const char *lbm_solver_error(const lbm_solver *s);
 /* reset(): init kernel, step counter 0 (src/CLbmSolver.hpp:619-635) */

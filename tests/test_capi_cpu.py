@@ -130,3 +130,14 @@ def test_required_bytes_do_not_depend_on_launch_shape():
                 assert lib.lbm_solver_required_bytes(C.byref(cfg), C.byref(n)) == _capi.LBM_OK
                 sizes.add(n.value)
         assert len(sizes) == 1, (dims, sizes)
+
+
+def test_colocate_rejects_bad_arguments_without_device():
+    """lbm_solver_colocate checks its arguments before any device call."""
+    import ctypes as C
+
+    lib = _capi.load_library()
+    assert lib.lbm_solver_colocate(None, 0) == _capi.LBM_ERR_INVALID_ARGUMENT
+    arr = (C.c_void_p * 2)(None, None)
+    assert lib.lbm_solver_colocate(arr, 2) == _capi.LBM_ERR_INVALID_ARGUMENT
+    assert "null" in lib.lbm_last_error().decode()

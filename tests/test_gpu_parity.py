@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 import turbulent_lbm_multigpu_amd as lbm  # noqa: E402,F401
 from turbulent_lbm_multigpu_amd import decomposition as dec  # noqa: E402
-from turbulent_lbm_multigpu_amd.solver import LbmGroup, LbmSolver  # noqa: E402
+from turbulent_lbm_multigpu_amd.solver import LbmGroup, LbmSolver, colocate  # noqa: E402
 from turbulent_lbm_multigpu_amd import _capi  # noqa: E402
 
 RTOL_RHO_U = 1e-5
@@ -194,19 +194,19 @@ def test_mass_energy_fp64_reduction():
 
 
 def build_gpu_group(domain_size, nums, *, lid, amp, seed, storage=_capi.STORAGE_F32, mode=0, graph=True,
-                    periodic=(False, False, False), **kw):
+                    periodic=(False, False, False), colocated=False, **kw):
     mono = dec.monolithic_size(domain_size, nums)
     subs = dec.decompose(domain_size, nums, periodic=periodic)
-    solvers = []
-    for sd in subs:
-        bc = [sd.bc[0][0], sd.bc[0][1], sd.bc[1][0], sd.bc[1][1], sd.bc[2][0], sd.bc[2][1]]
-        s = LbmSolver(sd.domain.size, bc, lid_velocity=lid, storage=storage, store_velocity=True,
-                      store_density=True, **kw)
+    solvers = [LbmSolver(sd.domain.size, [sd.bc[0][0], sd.bc[0][1], sd.bc[1][0], sd.bc[1][1], sd.bc[2][0],
+                                          sd.bc[2][1]], lid_velocity=lid, storage=storage, store_velocity=True,
+                         store_density=True, **kw) for sd in subs]
+    if colocated:
+        colocate(solvers)
+    for sd, s in zip(subs, solvers):
         s.init_perturbed(amp, seed, 0, dec.key_origin(sd), mono)
         if sd.has_lid:
             o, sz = dec.lid_region(sd.domain.size)
             s.set(_capi.FIELD_FLAGS, np.full(sz[::-1], orc.FLAG_INJECTION, np.int32), o, sz)
-        solvers.append(s)
     grp = LbmGroup(solvers)
     for sd in subs:
         for c in sd.comms:
@@ -216,22 +216,67 @@ def build_gpu_group(domain_size, nums, *, lid, amp, seed, storage=_capi.STORAGE_
     return grp, subs, mono
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_group_2x2x2_matches_oracle_decomposition(graph):
-    say(f"case: group 2x2x2 of 14^3 graph={graph}")
+@pytest.mark.parametrize("graph,colocated", [(True, False), (False, False), (True, True)])
+def test_group_2x2x2_matches_oracle_decomposition(graph, colocated):
+    """Also with the eight subdomains moved into one allocation
+    (lbm_solver_colocate): distributions, velocity and density outputs."""
+    say(f"case: group 2x2x2 of 14^3 graph={graph} colocated={colocated}")
     from tests.test_oracle import build_oracle_decomposition
     domain_size, nums = (28, 28, 28), (2, 2, 2)
     kw = dict(inv_tau=1 / 0.5005, tau=0.5005, smagorinsky_cs=0.17)
     multi, mono = build_oracle_decomposition(domain_size, nums, lid=0.1, amp=1e-4, seed=20260101, **kw)
-    grp, subs, _ = build_gpu_group(domain_size, nums, lid=0.1, amp=1e-4, seed=20260101, graph=graph, **kw)
+    grp, subs, _ = build_gpu_group(domain_size, nums, lid=0.1, amp=1e-4, seed=20260101, graph=graph,
+                                   colocated=colocated, **kw)
     for steps in (3, 6):   # odd then even totals: both sync kinds, graph replays in between
         multi.run(steps)
         grp.step(steps)
         grp.wait()
         for sub, s in zip(multi.subs, grp.solvers):
             np.testing.assert_array_equal(gpu_dd(s), sub["domain"].dd)
+            if colocated:
+                np.testing.assert_array_equal(s.store(_capi.FIELD_VELOCITY).ravel(), sub["domain"].vel.ravel())
+                np.testing.assert_array_equal(s.store(_capi.FIELD_DENSITY).ravel(), sub["domain"].dens.ravel())
     grp.close()
-    for s in grp.solvers:
+    for s in grp.solvers[::-1]:   # the shared allocation goes with the last solver
+        s.close()
+
+
+def test_colocate_keeps_state_and_rejects_group_members():
+    """Colocation moves buffers with their contents (distributions, flags,
+    outputs); a solver already colocated or in a group is refused."""
+    say("case: colocate state and errors")
+    from turbulent_lbm_multigpu_amd._capi import LbmError
+    solvers = [LbmSolver(d, (1,) * 6, inv_tau=1 / 0.55, tau=0.55, smagorinsky_cs=0.17, lid_velocity=0.06,
+                         store_velocity=v, store_density=v) for d, v in
+               (((21, 17, 13), True), ((16, 16, 16), False), ((30, 9, 11), True))]
+    before = []
+    for k, s in enumerate(solvers):
+        s.init_perturbed(1e-3, 40 + k)
+        s.run(3 + k)
+        s.wait()
+        before.append((gpu_dd(s), s.store(_capi.FIELD_FLAGS), s.store(_capi.FIELD_VELOCITY) if k != 1 else None))
+    colocate(solvers)
+    for s, (dd, fl, vel) in zip(solvers, before):
+        np.testing.assert_array_equal(gpu_dd(s), dd)
+        np.testing.assert_array_equal(s.store(_capi.FIELD_FLAGS), fl)
+        if vel is not None:
+            np.testing.assert_array_equal(s.store(_capi.FIELD_VELOCITY), vel)
+    with pytest.raises(LbmError) as e:
+        colocate(solvers[:1])
+    assert e.value.code == _capi.LBM_ERR_STATE
+    extra = LbmSolver((12, 12, 12), (1,) * 6, inv_tau=1 / 0.6)
+    grp = LbmGroup([extra])
+    with pytest.raises(LbmError) as e:
+        colocate([extra])
+    assert e.value.code == _capi.LBM_ERR_STATE
+    grp.close()
+    extra.close()
+    # stepping after the move, then destroy in creation order
+    for s in solvers:
+        s.run(2)
+        s.wait()
+        assert np.isfinite(gpu_dd(s)).all()
+    for s in solvers:
         s.close()
 
 

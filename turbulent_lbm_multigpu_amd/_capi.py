@@ -66,7 +66,7 @@ EXPORTED_SYMBOLS = (
     "lbm_group_wait", "lbm_group_set_timing", "lbm_group_get_timing", "lbm_group_sync",
     "lbm_group_get_timing_detail", "lbm_solver_place_spheres", "lbm_group_set_launch",
     "lbm_group_get_launch", "lbm_group_autotune",
-    "lbm_solver_required_bytes", "lbm_solver_create_in", "lbm_group_create_empty", "lbm_group_add_member",
+    "lbm_solver_required_bytes", "lbm_solver_create_in", "lbm_solver_colocate", "lbm_group_create_empty", "lbm_group_add_member",
     "lbm_group_remove_member", "lbm_group_member_count", "lbm_group_set_member_stop", "lbm_group_get_stream",
     "lbm_server_config_default", "lbm_case_default", "lbm_server_create", "lbm_server_destroy",
     "lbm_server_error", "lbm_server_case_bytes", "lbm_server_submit", "lbm_server_progress",
@@ -237,6 +237,7 @@ def load_library(path: str | None = None):
     sig("lbm_group_autotune", I, P, I, IP, IP, C.POINTER(D))
     sig("lbm_solver_required_bytes", I, C.POINTER(SolverConfig), C.POINTER(C.c_size_t))
     sig("lbm_solver_create_in", I, C.POINTER(SolverConfig), P, C.c_size_t, P, C.POINTER(P))
+    sig("lbm_solver_colocate", I, C.POINTER(P), I)
     sig("lbm_group_create_empty", I, I, I, I, I, C.POINTER(P))
     sig("lbm_group_add_member", I, P, P, IP)
     sig("lbm_group_remove_member", I, P, I)

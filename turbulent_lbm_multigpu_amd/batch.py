@@ -25,7 +25,7 @@ import numpy as np
 
 from . import _capi
 from . import decomposition as dec
-from .solver import LbmGroup, LbmSolver
+from .solver import LbmGroup, LbmSolver, colocate
 
 
 @dataclass
@@ -66,13 +66,16 @@ def _restart(solvers: Sequence[LbmSolver], cases: Sequence[Cavity]):
 
 
 def run_batched(cases: Sequence[Cavity], steps: int, *, autotune_pairs: int = 2, storage: int = _capi.STORAGE_F32,
-                keep: bool = False, log=None):
-    """All cases in one group.  The autotuner times real steps of the batch
-    (12 launch shapes x (pairs + 1) step pairs); the cases are then restarted
-    from their initial state and all `steps` run as one graph launch (one
-    fused step-kernel node per half-step).  Tuning and restart are counted in
-    the wall time."""
+                keep: bool = False, colocated: bool = False, log=None):
+    """All cases in one group (their buffers moved side by side into one
+    allocation if `colocated`).  The autotuner times real steps of the
+    batch (12 launch shapes x (pairs + 1) step pairs); the cases are then
+    restarted from their initial state and all `steps` run as one graph
+    launch (one fused step-kernel node per half-step).  Tuning and restart
+    are counted in the wall time."""
     solvers = [build_solver(c, storage=storage) for c in cases]
+    if colocated:
+        colocate(solvers)
     g = LbmGroup(solvers)
     cells = sum(s.cells for s in solvers)
     t0 = time.perf_counter()
@@ -89,7 +92,7 @@ def run_batched(cases: Sequence[Cavity], steps: int, *, autotune_pairs: int = 2,
     g.wait()
     t2 = time.perf_counter()
     after = g.launch_counts()
-    res = {"cases": len(cases), "steps": steps, "cells": cells, "tune_s": round(t1 - t0, 4),
+    res = {"cases": len(cases), "steps": steps, "cells": cells, "colocated": colocated, "tune_s": round(t1 - t0, 4),
            "run_s": round(t2 - t1, 4), "wall_s": round(t2 - t0, 4),
            "cases_per_s": round(len(cases) / (t2 - t0), 3),
            "mlups": round(cells * steps / (t2 - t0) / 1e6, 1),
@@ -136,6 +139,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--autotune-pairs", type=int, default=2)
     ap.add_argument("--sequential", action="store_true", help="also time the one-case-at-a-time path")
+    ap.add_argument("--colocate", action="store_true",
+                    help="move every case into one shared allocation (default: allocations of their own)")
     a = ap.parse_args(argv)
 
     def log(m):
@@ -144,7 +149,8 @@ def main(argv=None):
     cases = make_cavities(a.cases, a.seed)
     out = {"workload": f"{a.cases} LES cavities, edges U{{16..96}}, lid U[0.02,0.1], tau lognormal(0.52, 0.01) "
                        f">= 0.5002, Cs 0.17, FP32, {a.steps} steps", "batched": run_batched(
-                           cases, a.steps, autotune_pairs=a.autotune_pairs, log=log)[0]}
+                           cases, a.steps, autotune_pairs=a.autotune_pairs, colocated=a.colocate,
+                           log=log)[0]}
     if a.sequential:
         out["sequential"] = run_sequential(cases, a.steps, log=log)[0]
         out["speedup"] = round(out["batched"]["cases_per_s"] / out["sequential"]["cases_per_s"], 3)

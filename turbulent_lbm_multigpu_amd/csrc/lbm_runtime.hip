@@ -352,6 +352,13 @@ uint32_t norm_mask(const int *norm) {
 
 // ---- objects ----------------------------------------------------------------
 
+// One device allocation holding the buffers of several solvers
+// (lbm_solver_colocate); the last of them to be destroyed frees it.
+struct SharedBlock {
+    void *base = nullptr;
+    int refs = 0;
+};
+
 struct lbm_solver {
     lbm_solver_config cfg{};
     long long cells = 0, padded = 0;  // padded: cells rounded up to whole layout blocks
@@ -375,6 +382,7 @@ struct lbm_solver {
     unsigned version = 0;      // bumps when kernel parameters change
     lbm_group *group = nullptr;
     bool external = false;     // buffers live in caller-provided device memory
+    SharedBlock *block = nullptr;  // buffers live in a block shared with other solvers (lbm_solver_colocate)
     long long stop_at = std::numeric_limits<long long>::max();  // group half-step index of its freeze
 };
 
@@ -783,7 +791,13 @@ extern "C" int lbm_solver_destroy(lbm_solver *s) {
     if (!s) return LBM_OK;
     if (s->group) return fail(s, LBM_ERR_STATE, "destroy the group before its member solvers");
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    if (!s->external) {
+    if (s->block) {
+        (void)hipFree(s->staging);  // allocated on its own when a transfer needed it
+        if (--s->block->refs == 0) {
+            (void)hipFree(s->block->base);
+            delete s->block;
+        }
+    } else if (!s->external) {
         (void)hipFree(s->dd);
         (void)hipFree(s->flags);
         (void)hipFree(s->vel);
@@ -795,6 +809,77 @@ extern "C" int lbm_solver_destroy(lbm_solver *s) {
     if (s->d_partials) (void)hipFree(s->d_partials);
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
     delete s;
+    return LBM_OK;
+}
+
+extern "C" int lbm_solver_colocate(lbm_solver *const *solvers, int n) {
+    if (!solvers || n <= 0) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "lbm_solver_colocate: no solvers");
+    std::vector<size_t> off((size_t)n);
+    size_t total = 0;
+    const int device = solvers[0] ? solvers[0]->cfg.device : -1;
+    for (int i = 0; i < n; ++i) {
+        lbm_solver *s = solvers[i];
+        const std::string who = "lbm_solver_colocate: solver " + std::to_string(i);
+        if (!s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, who + " is null");
+        for (int k = 0; k < i; ++k)
+            if (solvers[k] == s) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, who + " is listed twice");
+        if (s->group) return record(nullptr, LBM_ERR_STATE, who + " is a group member: colocate before grouping");
+        if (s->external || s->block)
+            return record(nullptr, LBM_ERR_STATE, who + " already lives in a shared or caller-owned block");
+        if (s->cfg.device != device) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, who + " is on another device");
+        // the persistent buffers of the solver's layout; staging stays separate
+        off[(size_t)i] = total;
+        total += solver_layout(s->cfg, s->cells, s->padded).staging;
+    }
+    if (hipSetDevice(device) != hipSuccess) return record(nullptr, LBM_ERR_DEVICE, "hipSetDevice failed");
+    void *base = nullptr;
+    if (hipMalloc(&base, total) != hipSuccess)
+        return record(nullptr, LBM_ERR_OUT_OF_MEMORY,
+                      "lbm_solver_colocate: allocation of " + std::to_string(total) + " bytes failed");
+    SharedBlock *blk = new SharedBlock{base, 0};
+    for (int i = 0; i < n; ++i) {
+        lbm_solver *s = solvers[i];
+        const SolverLayout L = solver_layout(s->cfg, s->cells, s->padded);
+        char *b = (char *)base + off[(size_t)i];
+        const size_t es = elem_size(s->cfg.storage), cs = compute_size(s->cfg.storage);
+        void *dd = b + L.dd, *vel = s->vel ? b + L.vel : nullptr, *dens = s->dens ? b + L.dens : nullptr;
+        uint8_t *flags = (uint8_t *)(b + L.flags);
+        int *tile = (int *)(b + L.tile);
+        hipError_t e = hipMemcpyAsync(dd, s->dd, (size_t)19 * s->padded * es, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(flags, s->flags, (size_t)s->cells, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess && vel)
+            e = hipMemcpyAsync(vel, s->vel, (size_t)3 * s->cells * cs, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess && dens)
+            e = hipMemcpyAsync(dens, s->dens, (size_t)s->cells * cs, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(tile, s->d_tile, sizeof(int) * (size_t)s->ntiles, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) {
+            // solvers before this one have moved and hold the block; this one is unchanged
+            const std::string m = std::string("lbm_solver_colocate: copy failed: ") + hipGetErrorString(e);
+            if (blk->refs == 0) {
+                (void)hipFree(base);
+                delete blk;
+            }
+            return record(nullptr, LBM_ERR_DEVICE, m);
+        }
+        (void)hipFree(s->dd);
+        (void)hipFree(s->flags);
+        (void)hipFree(s->vel);
+        (void)hipFree(s->dens);
+        (void)hipFree(s->d_desc);
+        (void)hipFree(s->d_tile);
+        s->dd = dd;
+        s->flags = flags;
+        s->vel = vel;
+        s->dens = dens;
+        s->d_desc = (DomDev *)(b + L.desc);
+        s->d_tile = tile;
+        s->block = blk;
+        blk->refs++;
+        const int rc = upload_desc(s);
+        if (rc) return rc;
+    }
     return LBM_OK;
 }
 

@@ -241,6 +241,19 @@ def copy_box(src: LbmSolver, src_origin, dst: LbmSolver, dst_origin, size, norm=
         raise LbmError(rc, dst._err())
 
 
+def colocate(solvers: Sequence[LbmSolver]):
+    """Moves the solvers' device buffers into one allocation, side by side
+    (lbm_solver_colocate): the subdomains of one decomposition step faster
+    that way than in separate allocations.  Call before grouping them."""
+    if not solvers:
+        return
+    lib = _capi.load_library()
+    arr = (C.c_void_p * len(solvers))(*[s.handle for s in solvers])
+    rc = lib.lbm_solver_colocate(arr, len(solvers))
+    if rc != 0:
+        raise LbmError(rc, lib.lbm_last_error().decode())
+
+
 class LbmGroup:
     """Subdomains stepped by one fused kernel per half-step plus on-device
     ghost-layer exchange (x faces, then y, then z)."""
