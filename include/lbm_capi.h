@@ -216,13 +216,17 @@ int lbm_group_set_exchange_mode(lbm_group *g, int mode);
  * lines or periodic rings); one work-group holds the same rows of every
  * member of a chain and hands the face values over in LDS, so the x-face copy
  * launch disappears (the x faces are the strided ones: one element per cache
- * line).  Every slot of every member is identical to the copy-based
- * exchange.  Chains whose rows (or, for wide rows, whose 64-cell face runs)
- * do not fit the work-group keep their copies.  lbm_group_sync still
- * copies.  `mode`: 0 = copy launch, 1 = inside the step kernel,
- * LBM_FUSED_X_AUTO = inside the step kernel for FP32/FP64 storage and the copy
- * launch for FP16/BF16 (measured faster for the two-byte storages, whose
- * kernels lose more to the handoff tiles than the copy launch costs). */
+ * line).  A periodic ring of one member in a two-byte storage launched with
+ * packed pairs (kernel variants 8, 11) and rows of an even length >= 128
+ * instead keeps each row's seam in one wavefront (cells 0..63 and X-64..X-1)
+ * and moves the face values by lane reads.  Every slot of every member is
+ * identical to the copy-based exchange.  Chains whose rows (or, for wide
+ * rows, whose 64-cell face runs) do not fit the work-group keep their
+ * copies.  lbm_group_sync still copies.  `mode`: 0 = copy launch, 1 = inside
+ * the step kernel, LBM_FUSED_X_AUTO = inside the step kernel for FP32/FP64
+ * storage; for FP16/BF16 only the seam wavefronts, other x faces by the copy
+ * launch (measured: the two-byte kernels lose more to the handoff tiles than
+ * the copy launch costs). */
 #define LBM_FUSED_X_AUTO 2
 int lbm_group_set_fused_x(lbm_group *g, int mode);
 /* replay the step pairs as hipGraphs (1) or launch eagerly (0) */

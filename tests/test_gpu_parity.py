@@ -18,6 +18,7 @@ import turbulent_lbm_multigpu_amd as lbm  # noqa: E402,F401
 from turbulent_lbm_multigpu_amd import decomposition as dec  # noqa: E402
 from turbulent_lbm_multigpu_amd.solver import LbmGroup, LbmSolver, colocate  # noqa: E402
 from turbulent_lbm_multigpu_amd import _capi  # noqa: E402
+from turbulent_lbm_multigpu_amd import batch  # noqa: E402
 
 RTOL_RHO_U = 1e-5
 
@@ -369,15 +370,15 @@ def test_full_size_precision_channel_fused_equals_copies(storage):
 
 @pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
 def test_full_size_precision_channel_packed_pairs(storage):
-    """The precision workload at its full size with the two-byte storages'
-    default launch (two cells per lane as packed 32-bit words, x faces by the
-    copy launch) against one cell per lane with the x faces in the kernel:
-    every stored bit identical after 7 steps."""
+    """The precision workload at its full size with two cells per lane as
+    packed 32-bit words and the x faces by the copy launch, against one cell
+    per lane with the x faces in the kernel, and against packed pairs with
+    the x faces in the kernel: every stored bit identical after 7 steps."""
     say(f"case: full-size channel 384x192x192 storage {storage}, packed pairs vs one cell per lane")
     from turbulent_lbm_multigpu_amd import precision
 
     out = []
-    for variant, fused in ((11, 0), (3, 1)):
+    for variant, fused in ((11, 0), (3, 1), (11, 1)):
         s, g, _ = precision.build_channel((384, 192, 192), storage, kernel_variant=variant, fused_x=fused)
         g.step(7)
         g.wait()
@@ -385,22 +386,26 @@ def test_full_size_precision_channel_packed_pairs(storage):
         g.close()
         s.close()
     np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
 
 
 @pytest.mark.parametrize("storage", [_capi.STORAGE_F32, _capi.STORAGE_F64, _capi.STORAGE_F16_SHIFTED,
                                      _capi.STORAGE_BF16_SHIFTED])
-def test_fused_x_auto_picks_per_storage(storage):
+@pytest.mark.parametrize("dims,variant", [((40, 18, 16), 3), ((160, 12, 10), 11)])
+def test_fused_x_auto_picks_per_storage(storage, dims, variant):
     """LBM_FUSED_X_AUTO: the periodic seam of a channel goes inside the step
-    kernel for FP32/FP64 (no exchange launch) and through the copy launch
-    for FP16/BF16 (one per step); the fields equal both
-    explicit modes bit for bit. Other mode values are rejected."""
-    say(f"case: fused x auto, channel (40,18,16) storage {storage}")
+    kernel for FP32/FP64 (no exchange launch); for FP16/BF16 it goes inside
+    the kernel as seam wavefronts when the launch packs pairs and the rows
+    are at least 128 cells long, else through the copy launch (one per
+    step); the fields equal both explicit modes bit for bit. Other mode
+    values are rejected."""
+    say(f"case: fused x auto, channel {dims} storage {storage} variant {variant}")
     from turbulent_lbm_multigpu_amd import precision
 
     out, launches = [], []
     for mode in (_capi.FUSED_X_AUTO, 1, 0):
-        s, g, _ = precision.build_channel((40, 18, 16), storage, tau=0.55, gx=2e-5, fill=0.04, rmin=2.0, rmax=3.0,
-                                          kernel_variant=3, fused_x=mode)
+        s, g, _ = precision.build_channel(dims, storage, tau=0.55, gx=2e-5, fill=0.04, rmin=2.0, rmax=3.0,
+                                          kernel_variant=variant, fused_x=mode)
         g.set_timing(True)
         g.step(8)
         g.wait()
@@ -412,9 +417,10 @@ def test_fused_x_auto_picks_per_storage(storage):
         g.close()
         s.close()
     two_byte = storage in (_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED)
+    seam = two_byte and variant == 11 and dims[0] >= 128
     # one launch per step (a step is one half-step kernel, alpha or beta)
     assert launches[1] == 0 and launches[2] == 8
-    assert launches[0] == (8 if two_byte else 0)
+    assert launches[0] == (8 if two_byte and not seam else 0)
     np.testing.assert_array_equal(out[0], out[1])
     np.testing.assert_array_equal(out[0], out[2])
 
@@ -506,6 +512,86 @@ def test_fused_x_storages_and_cache_policy(storage, variant):
     non-temporal cache policy: all slots equal to the copy exchange."""
     say(f"case: fused x 2x2x2 storage {storage} variant {variant}")
     fused_vs_copy((32, 28, 24), (2, 2, 2), 1, storage=storage, variant=variant, step_runs=(3, 4))
+
+
+@pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
+@pytest.mark.parametrize("domain_size,nums,periodic,wg", [
+    ((384, 10, 8), (1, 1, 1), True, 256),     # ring of one: each row's seam in one wavefront (3 per row)
+    ((384, 10, 8), (1, 1, 1), True, 128),     # ... two wavefronts per work-group
+    ((640, 10, 8), (1, 1, 1), True, 64),      # 5 wavefronts per row, one per work-group
+    ((128, 12, 10), (1, 1, 1), True, 256),    # the shortest seam row: one wavefront per row
+    ((200, 12, 10), (1, 1, 1), True, 192),    # a row of 100 pairs: the second wavefront half full
+    ((28, 12, 10), (1, 1, 1), True, 256),     # rows under 128 cells: one-cell chain tiles
+    ((56, 12, 10), (2, 1, 1), False, 256),    # a line of two: one-cell chain tiles
+    ((39, 12, 10), (3, 1, 1), False, 256),    # odd rows
+])
+def test_fused_x_packed_pairs(storage, domain_size, nums, periodic, wg):
+    """Two-byte storages with two cells per work-item (variants 8 and 11)
+    and the x faces inside the kernel: seam wavefronts for periodic rings of
+    one, one-cell chain tiles otherwise; every slot equal to the copy
+    exchange after odd and even step counts."""
+    for variant in (11, 8):
+        say(f"case: fused x packed pairs {domain_size} {nums} periodic {periodic} wg {wg} storage {storage} "
+            f"variant {variant}")
+        fused_vs_copy(domain_size, nums, 1, wg=wg, periodic=(periodic, False, False), storage=storage,
+                      variant=variant, step_runs=(3, 4))
+
+
+@pytest.mark.parametrize("domain_size,nums", [((42, 12, 10), (3, 1, 1)), ((768, 10, 8), (2, 1, 1))])
+def test_fused_x_packed_pairs_with_frozen_member(domain_size, nums):
+    """A frozen member inside a chain of a two-byte storage launched with
+    packed pairs (whole-row and run tiles): its slots as they stand feed its
+    partner, as the copies do."""
+    say(f"case: fused x packed pairs {domain_size} {nums} with a frozen member")
+    stops = ((1, 3), (2, 6)) if nums[0] == 3 else ((1, 3),)
+    fused_vs_copy(domain_size, nums, 1, stops=stops, step_runs=(5, 6), storage=_capi.STORAGE_F16_SHIFTED,
+                  variant=11)
+
+
+def test_seam_rings_in_a_batch_with_a_frozen_member():
+    """Seam rings of one in a batch: two periodic channels of packed pairs
+    (FP16), one frozen after 3 half-steps, next to a cavity, stepped in one
+    group: each equals the same case stepped alone with copies."""
+    say("case: seam rings in a batch with a frozen member")
+    from turbulent_lbm_multigpu_amd import precision
+
+    def channel(fused):
+        return precision.build_channel((160, 12, 10), _capi.STORAGE_F16_SHIFTED, tau=0.55, gx=2e-5, fill=0.04,
+                                       rmin=2.0, rmax=3.0, amp=1e-3, seed=5, kernel_variant=11, fused_x=fused)
+
+    solo = []
+    for stop in (None, 3):
+        s, g, _ = channel(0)
+        g.step(3 if stop else 8)
+        g.wait()
+        solo.append(s.store(_capi.FIELD_DD_RAW))
+        g.close()
+        s.close()
+    grp = LbmGroup.empty(_capi.STORAGE_F16_SHIFTED, 256, 11)
+    members = []
+    for stop in (None, 3):
+        s, g, _ = channel(0)   # set up with its own group (ghost planes synchronised), then moved
+        g.close()
+        idx = grp.add_member(s)
+        grp.add_comm(idx, idx, (1, 0, 0), (0, 0, 0), (1, 12, 10), (1, 0, 0))
+        grp.add_comm(idx, idx, (158, 0, 0), (159, 0, 0), (1, 12, 10), (-1, 0, 0))
+        if stop:
+            grp.set_member_stop(idx, stop)
+        members.append(s)
+    cav = batch.build_solver(batch.Cavity((20, 18, 16), 0.07, 0.53), storage=_capi.STORAGE_F16_SHIFTED,
+                             kernel_variant=11)
+    grp.add_member(cav)
+    grp.set_exchange_mode(1)
+    grp.set_fused_x(_capi.FUSED_X_AUTO)
+    grp.set_timing(True)
+    grp.step(8)
+    grp.wait()
+    assert grp.timing()["exchange_launches"] == 0   # both seams inside the kernel
+    for s, want in zip(members, solo):
+        np.testing.assert_array_equal(s.store(_capi.FIELD_DD_RAW), want)
+    grp.close()
+    for s in members + [cav]:
+        s.close()
 
 
 def test_fused_x_periodic_ring_of_two():

@@ -110,7 +110,15 @@ inline ChainShape chain_shape(int M, int X, int wg) {
 // The chains the joins form, each in x order: lines from the member with no
 // left partner, then rings.  A chain that does not fit (chain_shape with its
 // members' X) is left out and its members' joins are cleared.
-inline std::vector<std::vector<int>> find_chains(std::vector<XJoin> &j, const std::vector<int> &X, int wg) {
+// A periodic ring of one member in a kernel of packed pairs (two-byte storage,
+// two cells per work-item) keeps its seam inside one wavefront per row
+// (seam_pairs in lbm_device.hpp): rows of an even length of at least 128 cells.
+inline bool seam_ring_ok(bool pairs, int X) { return pairs && X % 2 == 0 && X >= 128; }
+// wavefronts per row of such a member: the seam wavefront, then runs of 64 pairs
+inline int seam_waves_per_row(int X) { return 1 + (X / 2 - 64 + 63) / 64; }
+
+inline std::vector<std::vector<int>> find_chains(std::vector<XJoin> &j, const std::vector<int> &X, int wg,
+                                                 bool pairs = false) {
     const int nm = (int)j.size();
     std::vector<char> seen((size_t)nm, 0);
     auto walk = [&](int start) {
@@ -127,7 +135,8 @@ inline std::vector<std::vector<int>> find_chains(std::vector<XJoin> &j, const st
     for (int i = 0; i < nm; ++i)
         if (!seen[i] && j[i].nb[0] >= 0) found.push_back(walk(i));
     for (auto &ch : found) {
-        if (chain_shape((int)ch.size(), X[ch[0]], wg).fits) {
+        const bool ring_of_one = ch.size() == 1 && j[ch[0]].nb[0] == ch[0];
+        if (chain_shape((int)ch.size(), X[ch[0]], wg).fits || (ring_of_one && seam_ring_ok(pairs, X[ch[0]]))) {
             kept.push_back(ch);
             continue;
         }

@@ -246,6 +246,7 @@ struct DomDev {
     double inv_tau, tau0, les_coeff, gx, gy, gz, lid;
     long long stop_at;  // launch half-step index from which this domain stays frozen
     unsigned xmask_a[2];  // slots this domain's x- / x+ ghost layer takes after alpha in a chain tile
+    int seam_wpr;         // > 0: periodic x seam inside the pair kernel, wavefronts per row (seam_pairs)
 };
 
 // ---- fused x chains ----------------------------------------------------------
@@ -997,6 +998,110 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
     }
 }
 
+// ---- periodic x seam of one domain, packed pairs ---------------------------
+// A two-byte domain whose x faces are each other's neighbours (a periodic
+// ring of one), stepped with two cells per work-item: every row is split into
+// `seam_wpr` wavefronts, the first holding pairs 0..31 and the row's last 32
+// pairs (cells 0..63 and X-64..X-1), the others the pairs in between.  The
+// x = 0 ghost cell (lane 0, low half) and its sender x = X-2 (lane 63, low
+// half), and the x = X-1 ghost (lane 63, high half) and its sender x = 1
+// (lane 0, high half), are then in one wavefront: the face values move by
+// lane reads, with no LDS, no barrier and no copy launch.  The ghost cells
+// take what the copy exchange would give them: after alpha the slots of
+// their receiver masks, in beta the populations they scatter into the inner
+// layer (e.x = +1 at x = 0, e.x = -1 at x = X-1).
+__device__ __forceinline__ unsigned pair_ld_w(const gchar *p, int s, int cells, bool wrap) {
+    if (!wrap) return pair_ld(p, (unsigned)s);
+    return (unsigned)raw_ld<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s, cells)) * 2u) |
+           ((unsigned)raw_ld<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s + 1, cells)) * 2u) << 16);
+}
+__device__ __forceinline__ void pair_st_w(gchar *p, int s, int cells, bool wrap, bool nt, unsigned v) {
+    if (!wrap) {
+        if (nt) pair_st<true>(p, (unsigned)s, v);
+        else pair_st<false>(p, (unsigned)s, v);
+        return;
+    }
+    raw_st<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s, cells)) * 2u, (unsigned short)v);
+    raw_st<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s + 1, cells)) * 2u, (unsigned short)(v >> 16));
+}
+
+template <class SP, bool BETA, bool NTS, bool NTL>
+__device__ __forceinline__ void seam_pairs(const DomDev *__restrict__ D, int tile) {
+    using T = typename SP::T;
+    const int X = D->nx, NP = X >> 1, wpr = D->seam_wpr;
+    const int cells = (int)D->cells;
+    const int gw = tile * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);  // wavefront of the domain
+    const int row = gw / wpr, k = gw - row * wpr;
+    if (row >= D->ny * D->nz) return;  // uniform per wavefront
+    const int l = (int)(threadIdx.x & 63);
+    const int p = k == 0 ? (l < 32 ? l : NP - 64 + l) : 32 + 64 * (k - 1) + l;
+    if (k != 0 && p >= NP - 32) return;  // past the row's interior pairs (never in the first wavefront)
+    const int g = row * X + 2 * p;
+    gchar *pl[19];
+    {
+        gchar *base = (gchar *)D->dd;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) pl[i] = base + (size_t)i * kBlk * SP::kBytes;
+    }
+    int d[19];
+    bool wrap = false;
+    if (BETA) {
+        const int XY = X * D->ny;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) d[i] = ex(i) + X * ey(i) + XY * ez(i);
+        const int dmax = 1 + X + XY;
+        wrap = row * X < dmax || row * X + X + dmax > cells;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) d[i] = 0;
+    }
+    // the pair's loads and collisions (as cells_pair2, with the linear wrap per half)
+    const unsigned fl = raw_ld<false, unsigned short>((const gchar *)D->flags, (unsigned)g);
+    const unsigned own = cell_base((unsigned)g) * 2u;
+    unsigned P[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        if (BETA && i < 18) P[i] = pair_ld_w(pl[opp(i)], g - d[i], cells, wrap);
+        else P[i] = raw_ld<NTL && !BETA, unsigned>(pl[i], own);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int flag = (int)((fl >> (8 * j)) & 0xffu);
+        T f[19], vx, vy, vz, dn;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) f[i] = kin<SP>((typename SP::U)(P[i] >> (16 * j)), i);
+        const bool wr = cell_update<T, BETA, SP::kShifted>(f, flag, D, vx, vy, vz, dn);
+        if (wr) {
+#pragma unroll
+            for (int q = 0; q < 19; ++q) {
+                const unsigned hv = kout<SP>(BETA ? f[q] : f[opp(q)], q);
+                P[q] = j ? ((P[q] & 0xffffu) | (hv << 16)) : ((P[q] & 0xffff0000u) | hv);
+            }
+        }
+        if (flag != kFlagGhost) store_macros<T>(D, (unsigned)(g + j), cells, vx, vy, vz, dn);
+    }
+    if (k == 0) {  // the seam: lane 0 holds cells 0, 1 and lane 63 cells X-2, X-1
+        const unsigned m0 = D->xmask_a[0], m1 = D->xmask_a[1];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+            const unsigned last = __builtin_amdgcn_readlane(P[q], 63), first = __builtin_amdgcn_readlane(P[q], 0);
+            const bool take0 = BETA ? ex(q) == 1 : ((m0 >> q) & 1u) != 0;
+            const bool take1 = BETA ? ex(q) == -1 : ((m1 >> q) & 1u) != 0;
+            if (l == 0 && take0) P[q] = (P[q] & 0xffff0000u) | (last & 0xffffu);
+            if (l == 63 && take1) P[q] = (P[q] & 0xffffu) | (first & 0xffff0000u);
+        }
+    }
+    // store addresses from a fresh copy of the cell index (see cells_pair2)
+    int gs = g;
+    asm volatile("" : "+v"(gs));
+    const unsigned own_st = cell_base((unsigned)gs) * 2u;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        if (BETA && q < 18) pair_st_w(pl[q], gs + d[q], cells, wrap, NTS && ex(q) == 0, P[q]);
+        else raw_st<NTS, unsigned>(pl[q], own_st, P[q]);
+    }
+}
+
 template <int ST, bool BETA, int VAR = 0>
 __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ doms, const int *__restrict__ tile_dom,
                                                     const XChain *__restrict__ chains, int xcd_map, long long *ctr) {
@@ -1027,6 +1132,12 @@ __global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ do
     }
     const DomDev *D = doms + td;
     if (h >= D->stop_at) return;  // a finished member of a continuous batch keeps its state
+    if constexpr (N == 2 && SP::kBytes == 2) {
+        if (D->seam_wpr > 0) {  // uniform per work-group
+            seam_pairs<SP, BETA, NTS, NTL>(D, t - D->tile_begin);
+            return;
+        }
+    }
     const int cells = (int)D->cells;
     const int block_cells = (int)blockDim.x * N;
     const int block_first = (t - D->tile_begin) * block_cells;

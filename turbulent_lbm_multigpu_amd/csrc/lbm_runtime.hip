@@ -410,6 +410,7 @@ struct lbm_group {
     bool descs_dirty = false;
     int exchange_mode = 0;
     bool fuse_x = false;  // x faces exchanged inside the step kernel where the geometry allows
+    bool fuse_auto = false;  // LBM_FUSED_X_AUTO: two-byte storages fuse only seam rings of packed pairs
     // [0]=after alpha, [1]=after beta; per phase x/y/z, and [3] the x-face copies
     // that the step kernel performs when fused (launched only by lbm_group_sync)
     CopyOp *d_ops[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
@@ -495,6 +496,7 @@ void fill_desc(lbm_solver *s) {
     d.gz = s->cfg.gravitation[2];
     d.lid = s->cfg.lid_velocity;
     d.stop_at = s->group ? s->stop_at : std::numeric_limits<long long>::max();
+    d.seam_wpr = 0;
 }
 
 int upload_desc(lbm_solver *s) {
@@ -1466,6 +1468,12 @@ static hipError_t grow_table(T *&ptr, size_t &cap, size_t n) {
     return e;
 }
 
+// The group's step kernels hold packed pairs: a two-byte storage launched with
+// two cells per work-item (kernel variants 8 and 11).
+static bool group_pairs(const lbm_group *g) {
+    return elem_size(g->storage) == 2 && cells_per_lane(g->variant & ~4) == 2;
+}
+
 // Descriptor and tile tables of the fused launch for the group's current
 // members and launch shape (work-group size and cells per lane); also
 // re-publishes the host half-step count to the device counter.
@@ -1499,6 +1507,20 @@ static int group_upload_descs(lbm_group *g) {
             continue;
         }
         if (pos[i] != 0) continue;  // a chain's tiles go with its first member
+        if (g->chains[c].size() == 1 && g->xfuse[i].nb[0] == (int)i &&
+            seam_ring_ok(group_pairs(g), g->members[i]->cfg.dims[0])) {
+            // a periodic ring of one in packed pairs: its seam stays in one
+            // wavefront per row (seam_pairs), tiles of whole wavefronts
+            const int X = g->members[i]->cfg.dims[0];
+            const long long waves =
+                (long long)seam_waves_per_row(X) * g->members[i]->cfg.dims[1] * g->members[i]->cfg.dims[2];
+            const int nt = (int)((waves + g->wg / 64 - 1) / (g->wg / 64));
+            descs[i].tile_begin = (int)tiles.size();
+            descs[i].ntiles = nt;
+            descs[i].seam_wpr = seam_waves_per_row(X);
+            tiles.insert(tiles.end(), (size_t)nt, (int)i);
+            continue;
+        }
         // a chain tile: the same R rows of all M members, each member on
         // ceil(R X / 64) wavefronts, M of those within the work-group and
         // R M <= kChainRows; the most rows that fit
@@ -1763,7 +1785,7 @@ extern "C" int lbm_group_set_fused_x(lbm_group *g, int mode) {
     if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
     if (mode != 0 && mode != 1 && mode != LBM_FUSED_X_AUTO)
         return gfail(g, LBM_ERR_INVALID_ARGUMENT, "fused-x mode must be 0, 1 or LBM_FUSED_X_AUTO");
-    if (mode == LBM_FUSED_X_AUTO) mode = elem_size(g->storage) >= 4;
+    g->fuse_auto = mode == LBM_FUSED_X_AUTO;
     g->fuse_x = mode != 0;
     g->plan_built = false;
     return LBM_OK;
@@ -1817,7 +1839,21 @@ static void plan_chains(lbm_group *g, const std::vector<char> &fuse) {
     g->xfuse = resolve_joins((int)g->members.size(), cand);
     std::vector<int> X;
     for (const lbm_solver *m : g->members) X.push_back(m->cfg.dims[0]);
-    g->chains = find_chains(g->xfuse, X, g->wg);
+    g->chains = find_chains(g->xfuse, X, g->wg, group_pairs(g));
+    if (g->fuse_auto && elem_size(g->storage) == 2) {
+        // two-byte storages: the chain tiles' handoff costs their kernels more
+        // than the copy launch does; the seam wavefront of a ring of one with
+        // packed pairs does not
+        std::vector<std::vector<int>> kept;
+        for (auto &ch : g->chains) {
+            if (ch.size() == 1 && g->xfuse[ch[0]].nb[0] == ch[0] && seam_ring_ok(group_pairs(g), X[ch[0]])) {
+                kept.push_back(ch);
+                continue;
+            }
+            for (int m : ch) g->xfuse[m] = XJoin{{-1, -1}, {0u, 0u}};
+        }
+        g->chains.swap(kept);
+    }
 }
 
 static int build_plan(lbm_group *g) {
