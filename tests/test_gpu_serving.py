@@ -164,22 +164,26 @@ def _served(cases, **kw):
         return [srv.fetch(i) for i in ids]
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_served_channel_equals_plain_channel(storage):
+@pytest.mark.parametrize("storage,dims,variant", [("f32", (30, 22, 18), 0), ("f16", (30, 22, 18), 0),
+                                                   ("f16", (160, 20, 18), 11)])
+def test_served_channel_equals_plain_channel(storage, dims, variant):
     """A served periodic channel (spheres, seeded perturbation, body force,
     an odd step count) against the channel built outside the server
     (precision.build_channel: periodic comms of the decomposition, ghost
-    planes synchronised once, FP32 seam inside the step kernel, FP16 seam by
-    the copy launch) and stepped by its own group: rho and j bit for bit."""
+    planes synchronised once, the seam by the copy launch) and stepped by its
+    own group: rho and j bit for bit.  The server puts the seam inside the
+    step kernel for FP32 and, with packed pairs and rows of 128 cells or
+    more, for FP16 (seam wavefronts)."""
     from turbulent_lbm_multigpu_amd import precision
 
-    dims, steps, seed, amp = (30, 22, 18), 37, 424242, 1e-4
-    say(f"case: served channel {dims} {storage}, {steps} steps vs build_channel")
+    steps, seed, amp = 37, 424242, 1e-4
+    say(f"case: served channel {dims} {storage} variant {variant}, {steps} steps vs build_channel")
     case = serving.make_case("channel", dims, steps, storage, tau=0.51, cs=0.17, body_force=1e-5, fill=0.05,
                              r_min=2.0, r_max=4.0, perturbation=amp, seed=seed)
-    (rho_s, j_s), = _served([case])
+    (rho_s, j_s), = _served([case], kernel_variant=variant)
     s, g, filled = precision.build_channel(dims, precision.STORAGES[storage], tau=0.51, cs=0.17, gx=1e-5,
-                                           sphere_seed=seed, fill=0.05, rmin=2.0, rmax=4.0, amp=amp, seed=seed)
+                                           sphere_seed=seed, fill=0.05, rmin=2.0, rmax=4.0, amp=amp, seed=seed,
+                                           fused_x=0)
     assert filled > 0.02
     g.step(steps)
     g.wait()
