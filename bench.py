@@ -98,6 +98,7 @@ def serving_leg(log, requests=100, rates=(6.0, 12.0, 24.0), seed=20260101):
     return {"workload": f"{requests} cavity/channel requests (seed {seed}), cubic edge log-uniform 32-192, steps "
                         "U{500..5000}, storage FP32 or FP16-shifted per request, 200 GB budget, launch shape "
                         "autotuned per storage pool",
+            "batched_max_active_cells": serving.BATCH_CELLS,
             "fp16_share_of_requests": round(sum(r.storage == "f16" for r in mix) / len(mix), 3),
             "cases_per_s": sat["fp16_batched"]["cases_per_s"],
             "fp32_one_at_a_time_cases_per_s": sat["fp32_one_at_a_time"]["cases_per_s"],

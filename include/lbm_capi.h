@@ -312,6 +312,9 @@ typedef struct lbm_server_config {
     int kernel_variant;
     int autotune;          /* 1: tune the launch shape per storage on its first chunk */
     int backfill;          /* 1: admit later requests that fit when the head of the queue does not */
+    long long max_active_cells; /* cells resident at once, 0 = no limit: the queue is then admitted in
+                                   order (no backfill past a case that would exceed it) unless nothing
+                                   is resident, so large cases run beside few others */
 } lbm_server_config;
 
 typedef struct lbm_case {
@@ -342,6 +345,7 @@ typedef struct lbm_server_stats {
     size_t bytes_in_use, bytes_peak;
     int queued, active, ready;
     double device_ms;  /* sum over chunks of the device time of the fused launches */
+    long long cells_peak;  /* most cells resident at once (in the pools or admitted for the next chunk) */
 } lbm_server_stats;
 
 void lbm_server_config_default(lbm_server_config *cfg);

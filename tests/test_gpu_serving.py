@@ -210,3 +210,30 @@ def test_server_small_work_groups(wg, variant):
     for (r0, j0), (r1, j1) in zip(got, want):
         np.testing.assert_array_equal(r0, r1)
         np.testing.assert_array_equal(j0, j1)
+
+
+def test_resident_cell_limit_admits_in_order():
+    """max_active_cells: the server never holds more resident cells than the
+    limit, except one case larger than the limit on its own (admitted when
+    nothing else is resident), and every case's result is the one an
+    unlimited server gives."""
+    say("case: resident-cell limit")
+    cases = lambda: ([serving.make_case("cavity", (24, 24, 24), 14 + k, "f32", seed=k) for k in range(6)]
+                     + [serving.make_case("cavity", (40, 40, 40), 9, "f32", seed=9)]
+                     + [serving.make_case("channel", (26, 20, 18), 11, "f16", fill=0.03, r_min=2.0, r_max=3.0,
+                                          seed=10)])
+    limit = 3 * 24 ** 3
+    with serving.Server(2e9, chunk_ms=0.5, max_chunk_steps=4, max_active_cells=limit) as srv:
+        ids = [srv.submit(c) for c in cases()]
+        for _ in range(20000):
+            srv.progress()
+            if not srv.busy():
+                break
+        st = srv.stats()
+        assert st["completed"] == len(ids) and st["failed"] == 0
+        assert 0 < st["cells_peak"] <= max(limit, 40 ** 3)
+        got = [srv.fetch(i) for i in ids]
+    want = _served(cases())
+    for (r0, j0), (r1, j1) in zip(got, want):
+        np.testing.assert_array_equal(r0, r1)
+        np.testing.assert_array_equal(j0, j1)

@@ -114,6 +114,7 @@ class ServerConfig(C.Structure):
         ("kernel_variant", C.c_int),
         ("autotune", C.c_int),
         ("backfill", C.c_int),
+        ("max_active_cells", C.c_longlong),
     ]
 
 
@@ -160,6 +161,7 @@ class ServerStats(C.Structure):
         ("active", C.c_int),
         ("ready", C.c_int),
         ("device_ms", C.c_double),
+        ("cells_peak", C.c_longlong),
     ]
 
 

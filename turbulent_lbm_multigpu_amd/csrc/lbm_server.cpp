@@ -170,6 +170,15 @@ int active_count(const lbm_server *s) {
     return n;
 }
 
+// cells of the cases resident in the pools or initialised for the next chunk
+long long resident_cells(const lbm_server *s) {
+    long long n = 0;
+    for (const Pool &p : s->pools)
+        for (const Case *c : p.members) n += c->cells;
+    for (const Case *c : s->ready) n += c->cells;
+    return n;
+}
+
 int pool_open(lbm_server *s, int slot) {
     Pool &p = s->pools[slot];
     if (p.g) return LBM_OK;
@@ -341,9 +350,13 @@ int case_readback(lbm_server *s, Case *c) {
 }
 
 int admit(lbm_server *s) {
+    long long cells = s->cfg.max_active_cells > 0 ? resident_cells(s) : 0;
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         if (s->cfg.max_active > 0 && active_count(s) + (int)s->ready.size() >= s->cfg.max_active) break;
         Case *c = *it;
+        // the cell limit admits in order: a case that would exceed it waits,
+        // and so does everything behind it, unless nothing is resident
+        if (s->cfg.max_active_cells > 0 && cells > 0 && cells + c->cells > s->cfg.max_active_cells) break;
         size_t off = 0;
         if (!s->arena.alloc(c->bytes, off)) {
             if (!s->cfg.backfill) break;
@@ -363,7 +376,9 @@ int admit(lbm_server *s) {
         }
         c->admit_s = s->now();
         s->ready.push_back(c);
+        cells += c->cells;
     }
+    s->stats.cells_peak = std::max(s->stats.cells_peak, resident_cells(s));
     return LBM_OK;
 }
 
@@ -381,6 +396,7 @@ extern "C" void lbm_server_config_default(lbm_server_config *c) {
     c->kernel_variant = 0;
     c->autotune = 0;
     c->backfill = 1;
+    c->max_active_cells = 0;
 }
 
 extern "C" void lbm_case_default(lbm_case *c) {
@@ -414,7 +430,8 @@ extern "C" int lbm_server_case_bytes(const lbm_case *c, size_t *bytes) {
 extern "C" int lbm_server_create(const lbm_server_config *cfg, lbm_server **out) {
     if (!cfg || !out) return LBM_ERR_INVALID_ARGUMENT;
     *out = nullptr;
-    if (cfg->memory_budget < ((size_t)1 << 20) || cfg->max_active < 0 || !(cfg->chunk_ms > 0.0) ||
+    if (cfg->memory_budget < ((size_t)1 << 20) || cfg->max_active < 0 || cfg->max_active_cells < 0 ||
+        !(cfg->chunk_ms > 0.0) ||
         cfg->max_chunk_steps < 2 || !lbm_work_group_supported(cfg->work_group_size) ||
         !lbm_variant_supported(cfg->kernel_variant))
         return LBM_ERR_INVALID_ARGUMENT;
