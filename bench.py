@@ -7,18 +7,24 @@ subdomains) plus the ghost exchange that follows it, as in
 CController::computeNextStep (src/CController.hpp:385-391).  MLUPS counts
 the configured 256^3 cells per step (src/CController.hpp:446-452).
 
+The timed steps are one graph launch, captured before the timed region.
 Prints one JSON line (rank 0).  The roofline entry divides the algorithmic
 152 B per cell update (19 loads + 19 stores of FP32) by the measured average
-duration of the step kernel (events on the group's stream); its traffic is the
-HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in
-profiles/ when that was taken with the same launch configuration.  The CPU
-baseline times the C oracle (OpenMP) on a bounded number of steps of the same
-workload.  Two more workloads are reported beside the headline number:
-batching (512 uneven LES cavities, 1000 steps, one fused launch per
-half-step, autotuned shape, against one case at a time) and serving (a
-seeded cavity/channel request stream through the continuous-batching server:
-FP16-shifted storage with batching against FP32 one case at a time, cases/s
-at saturation and p50/p99 latency at a Poisson arrival rate).
+duration of the step kernel (events on the group's stream): `frac`; and the
+whole step (kernel, exchange, launches) by the same bytes: `frac_end_to_end`.
+Its traffic is the HBM bytes per launch from the rocprofv3
+FETCH_SIZE/WRITE_SIZE summary in profiles/ when that was taken with the same
+launch configuration.  The CPU baseline times the C oracle built for the host
+(-O3 -march=native, OpenMP) on a bounded sample of the same workload.  Three
+more workloads are reported beside the headline number:
+* batching: 512 uneven LES cavities, 1000 steps, one fused launch per
+  half-step and one graph launch for all steps, autotuned launch shape,
+  against one case at a time;
+* precision: the 384x192x192 LES channel stored as FP64, FP32, FP16(f - w)
+  and BF16(f - w), 10,000 steps, errors against the FP64 run;
+* serving: 100 seeded cavity/channel requests through the continuous-batching
+  server, five legs at saturation (storage and batching apart) and three
+  Poisson rates with p50/p99 latency, and the served error against FP64.
 """
 import argparse
 import json
