@@ -82,7 +82,7 @@ typedef struct lbm_solver_config {
     double lid_velocity;    /* lattice lid speed (x) of VELOCITY_INJECTION cells */
     int store_velocity;     /* kernels write the velocity field (STORE_VELOCITY) */
     int store_density;      /* kernels write the density field (STORE_DENSITY) */
-    int work_group_size;    /* threads per work-group of the step kernels: 64..256; 0 = 256 */
+    int work_group_size;    /* threads per work-group of the step kernels: 64..512, whole wavefronts; 0 = 256 */
     int device;             /* HIP device ordinal */
     int kernel_variant;     /* tuning bits: 1 non-temporal loads, 2 non-temporal stores, 4 no XCD tile map,
                                8 / 16 two / four x-consecutive cells per work-item; supported: 0-3, 8, 11
@@ -256,14 +256,15 @@ int lbm_group_wait(lbm_group *g);
 int lbm_group_set_timing(lbm_group *g, int enable);
 int lbm_group_get_timing(lbm_group *g, double *step_kernel_ms, long long *step_kernel_launches,
                          double *exchange_ms, long long *exchange_launches);
-/* launch shape of the fused step kernel: work-group size (64..256) and
+/* launch shape of the fused step kernel: work-group size (64..512) and
  * kernel variant (see lbm_solver_config.kernel_variant); results do not
  * depend on it */
 int lbm_group_set_launch(lbm_group *g, int work_group_size, int kernel_variant);
 int lbm_group_get_launch(const lbm_group *g, int *work_group_size, int *kernel_variant);
 /* autotuner (new): times `pairs` step pairs for each work-group size in
- * {64,128,256} x kernel variant in {0,3,8,11} (1 or 2 cells per lane, each
- * with the default or the non-temporal cache policy), keeps the fastest; the timed steps are real steps of the simulation (24 * (pairs + 1)
+ * {64,128,256,512} x kernel variant in {0,3,8,11} (1 or 2 cells per lane,
+ * each with the default or the non-temporal cache policy), keeps the
+ * fastest; the timed steps are real steps of the simulation (32 * (pairs + 1)
  * of them, plus one if the counter was odd) */
 int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step);
 /* the same averages split into beta and alpha step kernels */

@@ -107,9 +107,9 @@ def test_solver_group_and_server_accept_the_same_launch_shapes():
         if s:
             accepted.append(v)
     assert accepted == [0, 1, 2, 3, 4, 5, 6, 7, 8, 11, 12, 15, 16, 20]
-    for wg in (32, 64, 100, 128, 192, 256, 320):
+    for wg in (32, 64, 100, 128, 192, 256, 320, 384, 448, 512, 576, 1024):
         s, g, srv = _variant_accepted_by(lib, 11, wg)
-        assert s == g == srv == (wg in (64, 128, 192, 256))
+        assert s == g == srv == (wg in (64, 128, 192, 256, 320, 384, 448, 512))
 
 
 def test_required_bytes_do_not_depend_on_launch_shape():
@@ -119,7 +119,7 @@ def test_required_bytes_do_not_depend_on_launch_shape():
     sizes = set()
     for dims in ((32, 32, 32), (17, 23, 29), (96, 48, 48)):
         sizes.clear()
-        for wg in (64, 128, 192, 256):
+        for wg in (64, 128, 192, 256, 512):
             for v in (0, 3, 8, 11, 16):
                 cfg = _capi.SolverConfig()
                 lib.lbm_solver_config_default(C.byref(cfg))

@@ -66,7 +66,7 @@ VARIANTS = [0, 2, 4, 8, 16, 16 | 4]
 
 
 @pytest.mark.parametrize("steps", [1, 2, 3, 8])
-@pytest.mark.parametrize("wg,variant", [(64, 0), (256, 0), (128, 8), (256, 16), (64, 16)])
+@pytest.mark.parametrize("wg,variant", [(64, 0), (256, 0), (128, 8), (256, 16), (64, 16), (512, 3), (512, 8)])
 def test_bgk_cavity_bitwise(steps, wg, variant):
     say(f"case: bgk cavity 19x14x12 steps={steps} wg={wg} variant={variant}")
     o, s = make_pair((19, 14, 12), inv_tau=1 / 0.6, lid=0.05, wg=wg, amp=1e-3, seed=3, variant=variant)
@@ -493,7 +493,7 @@ def test_fused_x_faces_match_copy_exchange(mode, domain_size, nums):
     fused_vs_copy(domain_size, nums, mode)
 
 
-@pytest.mark.parametrize("wg", [64, 128, 256])
+@pytest.mark.parametrize("wg", [64, 128, 256, 512])
 @pytest.mark.parametrize("domain_size", [(56, 12, 10), (200, 12, 10), (260, 12, 10)])
 def test_fused_x_chain_fits_work_group_or_copies(wg, domain_size):
     """A chain tile holds whole rows of every member, or for wider rows the
@@ -684,7 +684,7 @@ def test_two_byte_storage_launch_shapes_identical(storage, dims):
     64/128/256, give bit-identical states."""
     say(f"case: two-byte storage {storage} dims {dims} launch shapes")
     out = []
-    for variant, wg in ((0, 256), (3, 256), (8, 256), (8, 64), (11, 256), (11, 128), (16, 128)):
+    for variant, wg in ((0, 256), (3, 256), (8, 256), (8, 64), (11, 256), (11, 128), (16, 128), (11, 512), (3, 512)):
         s = LbmSolver(dims, (1,) * 6, inv_tau=1 / 0.53, tau=0.53, smagorinsky_cs=0.17, lid_velocity=0.06,
                       storage=storage, kernel_variant=variant, work_group_size=wg)
         s.init_perturbed(1e-3, 17)

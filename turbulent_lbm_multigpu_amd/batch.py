@@ -69,7 +69,7 @@ def run_batched(cases: Sequence[Cavity], steps: int, *, autotune_pairs: int = 2,
                 keep: bool = False, colocated: bool = False, log=None):
     """All cases in one group (their buffers moved side by side into one
     allocation if `colocated`).  The autotuner times real steps of the
-    batch (12 launch shapes x (pairs + 1) step pairs); the cases are then
+    batch (16 launch shapes x (pairs + 1) step pairs); the cases are then
     restarted from their initial state and all `steps` run as one graph
     launch (one fused step-kernel node per half-step).  Tuning and restart
     are counted in the wall time."""

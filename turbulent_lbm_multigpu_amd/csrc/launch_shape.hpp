@@ -13,5 +13,5 @@ inline bool lbm_variant_supported(int variant) {
     return v == 0 || v == 1 || v == 2 || v == 3 || v == 8 || v == 11 || v == 16;
 }
 
-// work-items per work-group: whole wavefronts of 64, at most 256
-inline bool lbm_work_group_supported(int wg) { return wg >= 64 && wg <= 256 && wg % 64 == 0; }
+// work-items per work-group: whole wavefronts of 64, at most 512
+inline bool lbm_work_group_supported(int wg) { return wg >= 64 && wg <= 512 && wg % 64 == 0; }

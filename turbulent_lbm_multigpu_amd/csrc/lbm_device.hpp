@@ -1103,7 +1103,7 @@ __device__ __forceinline__ void seam_pairs(const DomDev *__restrict__ D, int til
 }
 
 template <int ST, bool BETA, int VAR = 0>
-__global__ void __launch_bounds__(256) step_kernel(const DomDev *__restrict__ doms, const int *__restrict__ tile_dom,
+__global__ void __launch_bounds__(512) step_kernel(const DomDev *__restrict__ doms, const int *__restrict__ tile_dom,
                                                     const XChain *__restrict__ chains, int xcd_map, long long *ctr) {
     using SP = Storage<ST>;
     constexpr bool NTS = (VAR & kVarNtStore) != 0;

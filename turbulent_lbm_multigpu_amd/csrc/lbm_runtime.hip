@@ -2182,7 +2182,7 @@ extern "C" int lbm_group_get_launch(const lbm_group *g, int *work_group_size, in
 }
 
 // Autotuner: times `pairs` beta+alpha step pairs (with their exchanges) for
-// every work-group size in {64, 128, 256} and every variant in {0, 3, 8, 11}
+// every work-group size in {64, 128, 256, 512} and every variant in {0, 3, 8, 11}
 // (1 or 2 cells per lane, each with the default or the non-temporal cache
 // policy) and keeps the fastest.  The steps it runs are
 // ordinary simulation steps: every launch shape computes the same
@@ -2200,7 +2200,7 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
         if (rc) return rc;
     }
     if (g->ntiles == 0) return gfail(g, LBM_ERR_STATE, "autotune needs at least one member");
-    const int wgs[3] = {64, 128, 256};
+    const int wgs[4] = {64, 128, 256, 512};
     const int vars[4] = {0, 3, 8, 11};
     double best = 1e300;
     int bw = g->wg, bv = g->variant;
@@ -2214,7 +2214,7 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
     } events_guard{&e0, &e1};
     GCHK(hipEventCreate(&e0));
     GCHK(hipEventCreate(&e1));
-    for (int wi = 0; wi < 3; ++wi)
+    for (int wi = 0; wi < 4; ++wi)
         for (int vi = 0; vi < 4; ++vi) {
             int rc = lbm_group_set_launch(g, wgs[wi], vars[vi] | (g->variant & 4));
             if (!rc) rc = group_prepare(g);
