@@ -518,6 +518,7 @@ def test_fused_x_storages_and_cache_policy(storage, variant):
 @pytest.mark.parametrize("domain_size,nums,periodic,wg", [
     ((384, 10, 8), (1, 1, 1), True, 256),     # ring of one: each row's seam in one wavefront (3 per row)
     ((384, 10, 8), (1, 1, 1), True, 128),     # ... two wavefronts per work-group
+    ((384, 10, 8), (1, 1, 1), True, 512),     # ... eight: a work-group ends inside a row
     ((640, 10, 8), (1, 1, 1), True, 64),      # 5 wavefronts per row, one per work-group
     ((128, 12, 10), (1, 1, 1), True, 256),    # the shortest seam row: one wavefront per row
     ((200, 12, 10), (1, 1, 1), True, 192),    # a row of 100 pairs: the second wavefront half full

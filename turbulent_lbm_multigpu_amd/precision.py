@@ -34,8 +34,10 @@ DEFAULT_VARIANT = {"f64": 3, "f32": 3, "f16": 11, "bf16": 11}
 def build_channel(dims, storage, *, tau=0.51, cs=0.17, gx=1e-6, sphere_seed=20260101, fill=0.05, rmin=3.0,
                   rmax=8.0, amp=0.0, seed=0, kernel_variant=0, work_group_size=256, fused_x=_capi.FUSED_X_AUTO):
     """One periodic channel in a group of one subdomain whose two x comms point
-    at itself (the periodic seam is a ghost-layer exchange, done inside the
-    step kernel with fused_x: 1, or _capi.FUSED_X_AUTO for FP32/FP64)."""
+    at itself.  The periodic seam is a ghost-layer exchange; with fused_x 1 or
+    _capi.FUSED_X_AUTO it runs inside the step kernel: as run tiles for one
+    cell per lane, as seam wavefronts for packed pairs (variant 8 or 11 of a
+    two-byte storage, AUTO only)."""
     sub = dec.decompose(dims, (1, 1, 1), periodic=(True, False, False))[0]
     bc = [sub.bc[0][0], sub.bc[0][1], sub.bc[1][0], sub.bc[1][1], sub.bc[2][0], sub.bc[2][1]]
     s = LbmSolver(dims, bc, inv_tau=1.0 / tau, tau=tau, smagorinsky_cs=cs, gravitation=(gx, 0.0, 0.0),
@@ -84,7 +86,7 @@ def run(dims=(384, 192, 192), steps=10000, storages=("f64", "f32", "f16", "bf16"
              "kinetic_energy": k1, "obstacle_fill": round(filled, 4)}
         if name == "f64":
             ref = (rho, u, fluid, m1, k1)
-        else:
+        elif ref is not None:   # errors only against an FP64 run earlier in `storages`
             rr, ur, fr, mr, kr = ref
             U = np.abs(ur[:, fr]).max()
             r.update(rho_max_abs_err=float(np.abs(rho[fr] - rr[fr]).max()),
@@ -106,10 +108,13 @@ def main():
     ap.add_argument("--variant", type=int, default=None,
                     help="kernel variant for every storage (default: 3 for f64/f32, 11 for f16/bf16)")
     ap.add_argument("--fused-x", type=int, default=_capi.FUSED_X_AUTO, choices=[0, 1, _capi.FUSED_X_AUTO],
-                    help="periodic x faces inside the step kernel (1), by a copy launch (0), or "
-                         "per storage (2: in the kernel for f32/f64, copy launch for f16/bf16)")
+                    help="periodic x faces by a copy launch (0), inside the step kernel (1), or per "
+                         "launch shape (2: run tiles for one cell per lane, seam wavefronts for packed "
+                         "pairs of a two-byte storage)")
+    ap.add_argument("--wg", type=int, default=256, help="work-group size (64..512, a multiple of 64)")
     a = ap.parse_args()
-    res = run(tuple(a.dims), a.steps, tuple(a.storages), kernel_variant=a.variant, fused_x=a.fused_x)
+    res = run(tuple(a.dims), a.steps, tuple(a.storages), kernel_variant=a.variant, fused_x=a.fused_x,
+              work_group_size=a.wg)
     print(json.dumps({"workload": f"LES channel {a.dims[0]}x{a.dims[1]}x{a.dims[2]}, periodic x, g_x 1e-6, "
                                   f"tau 0.51, Cs 0.17, 5% seeded spheres r 3-8, {a.steps} steps",
                       "results": res}))
