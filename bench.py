@@ -44,7 +44,7 @@ WORKLOAD = dict(domain=(256, 256, 256), nums=(2, 2, 2), cs=0.17, lid=0.1, tau=0.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernel_256cube_les_2x2x2_hbm_traffic.txt")
 
 
-def measured_traffic(exchange_mode, kernel_variant, wg, fused_x):
+def measured_traffic(exchange_mode, kernel_variant, wg, fused_x, wg_alpha=0):
     """HBM bytes per step-kernel launch from the committed counter summary,
     if it was measured with this launch configuration (else None)."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -55,8 +55,11 @@ def measured_traffic(exchange_mode, kernel_variant, wg, fused_x):
             cfg = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
         elif line.startswith("traffic_bytes_per_launch:"):
             val = float(line.split(":", 1)[1])
+    if cfg is None:
+        return None
+    cfg.setdefault("alpha_work_group", cfg.get("work_group"))
     if cfg != {"exchange_mode": str(exchange_mode), "kernel_variant": str(kernel_variant), "work_group": str(wg),
-               "fused_x": str(int(fused_x))}:
+               "fused_x": str(int(fused_x)), "alpha_work_group": str(wg_alpha or wg)}:
         return None
     return val
 
@@ -131,7 +134,7 @@ def precision_leg(log, dims=(384, 192, 192), steps=10000):
 
 
 def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_block=0, fused_x=True,
-                colocated=False):
+                colocated=False, wg_alpha=0):
     import numpy as np
 
     from turbulent_lbm_multigpu_amd import _capi
@@ -160,6 +163,8 @@ def build_group(exchange_mode=0, graph=True, wg=256, kernel_variant=0, lds_per_b
             grp.add_comm(sd.uid, c.dst_id, c.send_origin, c.recv_origin, c.send_size, c.comm_direction)
     grp.set_exchange_mode(exchange_mode)
     grp.set_fused_x(fused_x)
+    if wg_alpha:
+        grp.set_alpha_work_group(wg_alpha)
     grp.set_graph(graph)
     return grp
 
@@ -261,7 +266,7 @@ def copy_probe(lib, nbytes=1 << 30, reps=20):
     return best
 
 
-def main():
+def arg_parser() -> argparse.ArgumentParser:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
@@ -269,6 +274,8 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=200)
     ap.add_argument("--exchange-mode", type=int, default=1)
     ap.add_argument("--wg", type=int, default=256)
+    ap.add_argument("--wg-alpha", type=int, default=512,
+                    help="alpha's work-group size (0: --wg); at 512 its chain tiles hold two rows per subdomain")
     ap.add_argument("--kernel-variant", type=int, default=3)
     ap.add_argument("--lds-per-block", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
@@ -284,7 +291,11 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=100)
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = arg_parser().parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every subdomain runs on a single device "
@@ -297,7 +308,7 @@ def main():
     print(f"[bench] building 256^3 LES 2x2x2 (exchange_mode={args.exchange_mode}, wg={args.wg})",
           file=sys.stderr, flush=True)
     grp = build_group(args.exchange_mode, not args.no_graph, args.wg, args.kernel_variant,
-                      args.lds_per_block, bool(args.fused_x), args.colocate)
+                      args.lds_per_block, bool(args.fused_x), args.colocate, args.wg_alpha)
     grp.set_graph_pairs(args.graph_pairs)
     # the plain-copy reference of the roofline, measured before the step loop
     # (untimed; it also brings the device out of idle)
@@ -332,7 +343,7 @@ def main():
     def log(m):
         print(m, file=sys.stderr, flush=True)
 
-    traffic = measured_traffic(args.exchange_mode, args.kernel_variant, args.wg, args.fused_x)
+    traffic = measured_traffic(args.exchange_mode, args.kernel_variant, args.wg, args.fused_x, args.wg_alpha)
     grp.close()
     for s in grp.solvers:
         s.close()
@@ -364,7 +375,8 @@ def main():
                         "with on-device ghost exchange",
             "cs": WORKLOAD["cs"], "lid": WORKLOAD["lid"], "tau0": WORKLOAD["tau"],
             "init": "f_eq(1, du), du ~ U(+-1e-4), seed 20260101",
-            "exchange_mode": args.exchange_mode, "work_group": args.wg, "graph": not args.no_graph,
+            "exchange_mode": args.exchange_mode, "work_group": args.wg,
+            "alpha_work_group": args.wg_alpha or args.wg, "graph": not args.no_graph,
             "kernel_variant": args.kernel_variant, "fused_x": bool(args.fused_x),
             "graph_pairs": args.graph_pairs, "colocated": args.colocate,
         },

@@ -261,6 +261,14 @@ int lbm_group_get_timing(lbm_group *g, double *step_kernel_ms, long long *step_k
  * depend on it */
 int lbm_group_set_launch(lbm_group *g, int work_group_size, int kernel_variant);
 int lbm_group_get_launch(const lbm_group *g, int *work_group_size, int *kernel_variant);
+/* alpha's launches at a work-group size of their own (64..512, 0 = the
+ * group's; new): chain tiles of the x-face exchange then hold as many rows
+ * per member as alpha's work-group takes (two 128-cell rows per member of a
+ * two-member chain at 512) while beta keeps the group's size.  The chains
+ * kept are those both sizes hold.  Results do not depend on it.  The
+ * autotuner leaves it as set.  get returns the size alpha launches with. */
+int lbm_group_set_alpha_work_group(lbm_group *g, int work_group_size);
+int lbm_group_get_alpha_work_group(const lbm_group *g, int *work_group_size);
 /* autotuner (new): times `pairs` step pairs for each work-group size in
  * {64,128,256,512} x kernel variant in {0,3,8,11} (1 or 2 cells per lane,
  * each with the default or the non-temporal cache policy), keeps the

@@ -141,3 +141,11 @@ def test_colocate_rejects_bad_arguments_without_device():
     arr = (C.c_void_p * 2)(None, None)
     assert lib.lbm_solver_colocate(arr, 2) == _capi.LBM_ERR_INVALID_ARGUMENT
     assert "null" in lib.lbm_last_error().decode()
+
+
+def test_alpha_work_group_rejects_null_group():
+    """lbm_group_set/get_alpha_work_group check their arguments first."""
+    lib = _capi.load_library()
+    assert lib.lbm_group_set_alpha_work_group(None, 512) == _capi.LBM_ERR_INVALID_ARGUMENT
+    w = C.c_int()
+    assert lib.lbm_group_get_alpha_work_group(None, C.byref(w)) == _capi.LBM_ERR_INVALID_ARGUMENT

@@ -458,14 +458,18 @@ def fluid_slots(s: LbmSolver):
 
 
 def fused_vs_copy(domain_size, nums, mode, *, wg=256, periodic=(False, False, False), stops=(), step_runs=(5, 8),
-                  storage=_capi.STORAGE_F32, variant=0):
-    """Two identical groups, one with x faces inside the step kernel: every
-    slot of every subdomain is bit-identical after each run of steps."""
+                  storage=_capi.STORAGE_F32, variant=0, alpha_wg=0, fuse=True):
+    """Two identical groups, one with x faces inside the step kernel (and,
+    with alpha_wg, alpha launched at that work-group size): every slot of
+    every subdomain is bit-identical after each run of steps."""
     kw = dict(inv_tau=1 / 0.52, tau=0.52, smagorinsky_cs=0.17, work_group_size=wg, storage=storage,
               kernel_variant=variant)
     ref, _, _ = build_gpu_group(domain_size, nums, lid=0.08, amp=1e-3, seed=11, mode=mode, periodic=periodic, **kw)
     fus, _, _ = build_gpu_group(domain_size, nums, lid=0.08, amp=1e-3, seed=11, mode=mode, periodic=periodic, **kw)
-    fus.set_fused_x(True)
+    fus.set_fused_x(True if fuse else 0)
+    if alpha_wg:
+        fus.set_alpha_work_group(alpha_wg)
+        assert fus.alpha_work_group() == alpha_wg and fus.launch()[0] == wg
     for member, at in stops:
         ref.set_member_stop(member, at)
         fus.set_member_stop(member, at)
@@ -503,6 +507,35 @@ def test_fused_x_chain_fits_work_group_or_copies(wg, domain_size):
     nums = (4, 1, 1) if domain_size[0] == 56 else (2, 1, 1)
     say(f"case: fused x chain {domain_size} {nums} wg {wg}")
     fused_vs_copy(domain_size, nums, 1, wg=wg, step_runs=(3, 4))
+
+
+@pytest.mark.parametrize("domain_size,nums,periodic,wg,alpha_wg,fuse,stops", [
+    ((256, 12, 10), (2, 1, 1), False, 256, 512, True, ()),    # 128-cell rows: alpha's tiles hold 2 rows per member
+    ((256, 12, 10), (2, 1, 1), False, 512, 256, True, ()),    # ... and the reverse
+    ((256, 12, 10), (2, 1, 1), True, 256, 512, True, ()),     # a periodic ring of two
+    ((256, 12, 10), (2, 1, 1), False, 256, 512, True, ((1, 5),)),  # a frozen member
+    ((56, 12, 10), (4, 1, 1), False, 64, 512, True, ()),      # short rows, many per tile
+    ((28, 28, 28), (2, 2, 2), False, 256, 128, True, ()),     # x, y and z faces
+    ((28, 28, 28), (2, 2, 2), False, 256, 512, False, ()),    # copy exchange, alpha's own tile table
+    ((520, 12, 10), (2, 1, 1), False, 512, 64, True, ()),     # the plan is made for the smaller size
+])
+def test_alpha_work_group_of_its_own(domain_size, nums, periodic, wg, alpha_wg, fuse, stops):
+    """Alpha launched at a work-group size other than beta's: all slots
+    equal to one work-group size and the copy exchange."""
+    say(f"case: alpha wg {alpha_wg} beta wg {wg} {domain_size} {nums} periodic {periodic} fuse {fuse} stops {stops}")
+    fused_vs_copy(domain_size, nums, 1, wg=wg, alpha_wg=alpha_wg, fuse=fuse, stops=stops,
+                  periodic=(periodic, False, False), step_runs=(3, 4))
+
+
+@pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
+def test_alpha_work_group_of_its_own_seam_pairs(storage):
+    """Packed pairs with the periodic seam in one wavefront per row, alpha
+    at 512 and beta at 256: all slots equal to the copy exchange."""
+    say(f"case: alpha wg 512 seam pairs storage {storage}")
+    fused_vs_copy((384, 10, 8), (1, 1, 1), 1, wg=256, alpha_wg=512, periodic=(True, False, False), storage=storage,
+                  variant=11, step_runs=(3, 4))
+    fused_vs_copy((384, 10, 8), (1, 1, 1), 1, wg=128, alpha_wg=512, periodic=(True, False, False), storage=storage,
+                  variant=11, step_runs=(3, 4))
 
 
 @pytest.mark.parametrize("storage", [_capi.STORAGE_F64, _capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])

@@ -65,7 +65,8 @@ EXPORTED_SYMBOLS = (
     "lbm_group_prepare_steps", "lbm_group_get_launch_counts", "lbm_group_step",
     "lbm_group_wait", "lbm_group_set_timing", "lbm_group_get_timing", "lbm_group_sync",
     "lbm_group_get_timing_detail", "lbm_solver_place_spheres", "lbm_group_set_launch",
-    "lbm_group_get_launch", "lbm_group_autotune",
+    "lbm_group_get_launch", "lbm_group_set_alpha_work_group", "lbm_group_get_alpha_work_group",
+    "lbm_group_autotune",
     "lbm_solver_required_bytes", "lbm_solver_create_in", "lbm_solver_colocate", "lbm_group_create_empty", "lbm_group_add_member",
     "lbm_group_remove_member", "lbm_group_member_count", "lbm_group_set_member_stop", "lbm_group_get_stream",
     "lbm_server_config_default", "lbm_case_default", "lbm_server_create", "lbm_server_destroy",
@@ -236,6 +237,8 @@ def load_library(path: str | None = None):
     sig("lbm_group_get_timing_detail", I, P, C.POINTER(D), C.POINTER(D), C.POINTER(D))
     sig("lbm_group_set_launch", I, P, I, I)
     sig("lbm_group_get_launch", I, P, IP, IP)
+    sig("lbm_group_set_alpha_work_group", I, P, I)
+    sig("lbm_group_get_alpha_work_group", I, P, IP)
     sig("lbm_group_autotune", I, P, I, IP, IP, C.POINTER(D))
     sig("lbm_solver_required_bytes", I, C.POINTER(SolverConfig), C.POINTER(C.c_size_t))
     sig("lbm_solver_create_in", I, C.POINTER(SolverConfig), P, C.c_size_t, P, C.POINTER(P))

@@ -391,18 +391,27 @@ struct GroupComm {
     int send_origin[3], recv_origin[3], size[3], normal[3];
 };
 
+// The launch tables of one work-group size: member descriptors (with their
+// tile ranges), the tile -> member or chain map, and the chain descriptors.
+struct LaunchTables {
+    DomDev *d_descs = nullptr;
+    size_t descs_cap = 0;
+    int *d_tiles = nullptr;
+    size_t tiles_cap = 0;
+    int ntiles = 0;
+    XChain *d_chains = nullptr;
+    size_t chains_cap = 0;
+};
+
 struct lbm_group {
     std::vector<lbm_solver *> members;
     std::vector<hipStream_t> saved_streams;
     hipStream_t stream = nullptr;
     int storage = 0, wg = 256, device = 0;
     int variant = 0, lds = 0;  // launch shape of the fused step kernel (tuner-selected)
+    int wg_alpha = 0;          // alpha's work-group size when it differs from wg (0: wg)
     std::string err;
-    DomDev *d_descs = nullptr;
-    size_t descs_cap = 0;
-    int *d_tiles = nullptr;
-    size_t tiles_cap = 0;
-    int ntiles = 0;
+    LaunchTables lt[2];        // [0]: at wg (beta, and alpha unless wg_alpha is set), [1]: alpha at wg_alpha
     long long *d_ctr = nullptr;  // device half-step counter (two slots, see step_kernel)
     std::vector<unsigned> seen_versions;
     std::vector<GroupComm> comms;
@@ -419,8 +428,6 @@ struct lbm_group {
     int nblocks[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     std::vector<XJoin> xfuse;                 // per member, from the plan
     std::vector<std::vector<int>> chains;     // joined members in x order, from the plan
-    XChain *d_chains = nullptr;
-    size_t chains_cap = 0;
     long long step_counter = 0;
     bool use_graph = true;
     int graph_pairs = 64;  // most beta+alpha pairs one graph launch replays
@@ -1477,12 +1484,15 @@ static bool group_pairs(const lbm_group *g) {
 // Descriptor and tile tables of the fused launch for the group's current
 // members and launch shape (work-group size and cells per lane); also
 // re-publishes the host half-step count to the device counter.
-static int group_upload_descs(lbm_group *g) {
+// Fills the launch tables of the step kernel at work-group size wg: every
+// member's descriptor with its tile range, and the tile map (member index,
+// or -chain-1 for the tiles of a chain).
+static int build_tables(lbm_group *g, int wg, LaunchTables &T) {
     const size_t nm = g->members.size();
     std::vector<DomDev> descs(nm);
     std::vector<int> tiles;
     std::vector<XChain> xch;
-    const long long per_tile = (long long)g->wg * cells_per_lane(g->variant);
+    const long long per_tile = (long long)wg * cells_per_lane(g->variant);
     // members of a chain (plan): chain index and position
     std::vector<int> chain_of(nm, -1), pos(nm, -1);
     for (size_t c = 0; c < g->chains.size(); ++c)
@@ -1514,7 +1524,7 @@ static int group_upload_descs(lbm_group *g) {
             const int X = g->members[i]->cfg.dims[0];
             const long long waves =
                 (long long)seam_waves_per_row(X) * g->members[i]->cfg.dims[1] * g->members[i]->cfg.dims[2];
-            const int nt = (int)((waves + g->wg / 64 - 1) / (g->wg / 64));
+            const int nt = (int)((waves + wg / 64 - 1) / (wg / 64));
             descs[i].tile_begin = (int)tiles.size();
             descs[i].ntiles = nt;
             descs[i].seam_wpr = seam_waves_per_row(X);
@@ -1529,7 +1539,7 @@ static int group_upload_descs(lbm_group *g) {
         const lbm_solver *m0 = g->members[ch[0]];
         const int X = m0->cfg.dims[0];
         const long long rows = (long long)m0->cfg.dims[1] * m0->cfg.dims[2];
-        const ChainShape sh = chain_shape(M, X, g->wg);  // fits: the plan kept only chains that do
+        const ChainShape sh = chain_shape(M, X, wg);  // fits: the plan kept only chains that fit every wg
         XChain xc{};
         xc.nmem = M;
         xc.nx = X;
@@ -1555,16 +1565,27 @@ static int group_upload_descs(lbm_group *g) {
         tiles.insert(tiles.end(), (size_t)nt, -(int)xch.size() - 1);
         xch.push_back(xc);
     }
-    GCHK(grow_table(g->d_tiles, g->tiles_cap, tiles.size()));
-    GCHK(grow_table(g->d_descs, g->descs_cap, descs.size()));
-    if (!xch.empty()) GCHK(grow_table(g->d_chains, g->chains_cap, xch.size()));
-    g->ntiles = (int)tiles.size();
+    GCHK(grow_table(T.d_tiles, T.tiles_cap, tiles.size()));
+    GCHK(grow_table(T.d_descs, T.descs_cap, descs.size()));
+    if (!xch.empty()) GCHK(grow_table(T.d_chains, T.chains_cap, xch.size()));
+    T.ntiles = (int)tiles.size();
     if (!tiles.empty())
-        GCHK(hipMemcpyAsync(g->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, g->stream));
+        GCHK(hipMemcpyAsync(T.d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, g->stream));
     if (!xch.empty())
-        GCHK(hipMemcpyAsync(g->d_chains, xch.data(), sizeof(XChain) * xch.size(), hipMemcpyHostToDevice, g->stream));
+        GCHK(hipMemcpyAsync(T.d_chains, xch.data(), sizeof(XChain) * xch.size(), hipMemcpyHostToDevice, g->stream));
     if (!descs.empty())
-        GCHK(hipMemcpyAsync(g->d_descs, descs.data(), sizeof(DomDev) * descs.size(), hipMemcpyHostToDevice, g->stream));
+        GCHK(hipMemcpyAsync(T.d_descs, descs.data(), sizeof(DomDev) * descs.size(), hipMemcpyHostToDevice, g->stream));
+    // the host vectors are released on return: the copies have to finish first
+    GCHK(hipStreamSynchronize(g->stream));
+    return LBM_OK;
+}
+
+static bool alpha_split(const lbm_group *g) { return g->wg_alpha != 0 && g->wg_alpha != g->wg; }
+
+static int group_upload_descs(lbm_group *g) {
+    int rc = build_tables(g, g->wg, g->lt[0]);
+    if (!rc && alpha_split(g)) rc = build_tables(g, g->wg_alpha, g->lt[1]);
+    if (rc) return rc;
     const long long ctr[2] = {g->step_counter, g->step_counter};
     GCHK(hipMemcpyAsync(g->d_ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, g->stream));
     GCHK(hipStreamSynchronize(g->stream));
@@ -1669,9 +1690,11 @@ extern "C" int lbm_group_destroy(lbm_group *g) {
         m->group = nullptr;
         m->stop_at = std::numeric_limits<long long>::max();
     }
-    (void)hipFree(g->d_descs);
-    (void)hipFree(g->d_tiles);
-    (void)hipFree(g->d_chains);
+    for (LaunchTables &T : g->lt) {
+        (void)hipFree(T.d_descs);
+        (void)hipFree(T.d_tiles);
+        (void)hipFree(T.d_chains);
+    }
     (void)hipFree(g->d_ctr);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
@@ -1839,7 +1862,8 @@ static void plan_chains(lbm_group *g, const std::vector<char> &fuse) {
     g->xfuse = resolve_joins((int)g->members.size(), cand);
     std::vector<int> X;
     for (const lbm_solver *m : g->members) X.push_back(m->cfg.dims[0]);
-    g->chains = find_chains(g->xfuse, X, g->wg, group_pairs(g));
+    // chains that a work-group of either launch holds (beta at wg, alpha at wg_alpha)
+    g->chains = find_chains(g->xfuse, X, alpha_split(g) ? std::min(g->wg, g->wg_alpha) : g->wg, group_pairs(g));
     if (g->fuse_auto && elem_size(g->storage) == 2) {
         // two-byte storages: the chain tiles' handoff costs their kernels more
         // than the copy launch does; the seam wavefront of a ring of one with
@@ -1973,9 +1997,13 @@ static int group_prepare(lbm_group *g) {
 // tables' addresses (a table that grew was reallocated; its contents are read
 // at replay time and may change in place).
 static std::vector<long long> group_shape(const lbm_group *g) {
-    std::vector<long long> v = {g->ntiles, g->wg, g->variant, g->lds, (long long)(uintptr_t)g->d_descs,
-                                (long long)(uintptr_t)g->d_tiles, (long long)(uintptr_t)g->d_chains,
-                                (long long)(uintptr_t)g->d_ctr};
+    std::vector<long long> v = {g->wg, g->wg_alpha, g->variant, g->lds, (long long)(uintptr_t)g->d_ctr};
+    for (const LaunchTables &T : g->lt) {
+        v.push_back(T.ntiles);
+        v.push_back((long long)(uintptr_t)T.d_descs);
+        v.push_back((long long)(uintptr_t)T.d_tiles);
+        v.push_back((long long)(uintptr_t)T.d_chains);
+    }
     for (int k = 0; k < 2; ++k)
         for (int p = 0; p < 4; ++p) {
             v.push_back(g->nops[k][p]);
@@ -1999,7 +2027,9 @@ static hipError_t group_event_pair(lbm_group *g, int kind, hipEvent_t *a, hipEve
 }
 
 static int group_launch_half(lbm_group *g, bool beta, bool timed) {
-    if (g->ntiles == 0) return LBM_OK;
+    if (g->lt[0].ntiles == 0) return LBM_OK;
+    const bool split = !beta && alpha_split(g);
+    const LaunchTables &T = g->lt[split ? 1 : 0];
     // timing events belong to g->events from creation on, so every exit
     // path leaves them to drain_events / lbm_group_destroy
     hipEvent_t ea = nullptr, eb = nullptr;
@@ -2007,8 +2037,8 @@ static int group_launch_half(lbm_group *g, bool beta, bool timed) {
         GCHK(group_event_pair(g, beta ? 0 : 2, &ea, &eb));
         GCHK(hipEventRecord(ea, g->stream));
     }
-    GCHK(launch_step_kernel(g->storage, beta, g->variant, g->lds, g->ntiles, g->wg, g->d_descs, g->d_tiles,
-                            g->stream, g->d_ctr, g->d_chains));
+    GCHK(launch_step_kernel(g->storage, beta, g->variant, g->lds, T.ntiles, split ? g->wg_alpha : g->wg, T.d_descs,
+                            T.d_tiles, g->stream, g->d_ctr, T.d_chains));
     if (timed) GCHK(hipEventRecord(eb, g->stream));
     // the kernel just run was beta -> syncBeta, alpha -> syncAlpha (src/CController.hpp:385-391)
     const int k = beta ? 1 : 0;
@@ -2123,7 +2153,7 @@ extern "C" int lbm_group_step(lbm_group *g, int steps) {
     int rc = group_prepare(g);
     if (rc) return rc;
     const long long before = g->step_counter;
-    if (g->ntiles == 0) {  // no members: counters only (re-published with the next upload)
+    if (g->lt[0].ntiles == 0) {  // no members: counters only (re-published with the next upload)
         g->descs_dirty = true;
         group_advance_counters(g, before, steps);
         return LBM_OK;
@@ -2140,7 +2170,7 @@ extern "C" int lbm_group_prepare_steps(lbm_group *g, int steps) {
     if (steps < 0) return gfail(g, LBM_ERR_INVALID_ARGUMENT, "steps must be >= 0");
     GCHK(hipSetDevice(g->device));
     int rc = group_prepare(g);
-    if (rc || g->ntiles == 0) return rc;
+    if (rc || g->lt[0].ntiles == 0) return rc;
     int done = 0;
     return group_walk(g, steps, false, &done);
 }
@@ -2181,6 +2211,27 @@ extern "C" int lbm_group_get_launch(const lbm_group *g, int *work_group_size, in
     return LBM_OK;
 }
 
+// Alpha's launches at their own work-group size (0: the group's).  Chain
+// tiles then hold as many rows per member as alpha's work-group takes, while
+// beta keeps the group's; the plan keeps the chains that both sizes hold.
+extern "C" int lbm_group_set_alpha_work_group(lbm_group *g, int work_group_size) {
+    if (!g) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null group");
+    if (work_group_size != 0 && !lbm_work_group_supported(work_group_size))
+        return gfail(g, LBM_ERR_INVALID_ARGUMENT, "unsupported alpha work-group size");
+    GCHK(hipStreamSynchronize(g->stream));
+    g->wg_alpha = work_group_size;
+    group_drop_graph(g);
+    if (g->fuse_x) g->plan_built = false;  // the chains kept depend on both sizes
+    g->descs_dirty = true;
+    return g->plan_built ? group_upload_descs(g) : LBM_OK;
+}
+
+extern "C" int lbm_group_get_alpha_work_group(const lbm_group *g, int *work_group_size) {
+    if (!g || !work_group_size) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "null argument");
+    *work_group_size = g->wg_alpha ? g->wg_alpha : g->wg;
+    return LBM_OK;
+}
+
 // Autotuner: times `pairs` beta+alpha step pairs (with their exchanges) for
 // every work-group size in {64, 128, 256, 512} and every variant in {0, 3, 8, 11}
 // (1 or 2 cells per lane, each with the default or the non-temporal cache
@@ -2199,7 +2250,7 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
         int rc = group_prepare(g);
         if (rc) return rc;
     }
-    if (g->ntiles == 0) return gfail(g, LBM_ERR_STATE, "autotune needs at least one member");
+    if (g->lt[0].ntiles == 0) return gfail(g, LBM_ERR_STATE, "autotune needs at least one member");
     const int wgs[4] = {64, 128, 256, 512};
     const int vars[4] = {0, 3, 8, 11};
     double best = 1e300;

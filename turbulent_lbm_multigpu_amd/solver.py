@@ -353,6 +353,15 @@ class LbmGroup:
         self._check(self._lib.lbm_group_get_launch(self._h, C.byref(w), C.byref(v)))
         return w.value, v.value
 
+    def set_alpha_work_group(self, work_group_size: int):
+        """Alpha launches at this work-group size (0: the group's); beta keeps the group's."""
+        self._check(self._lib.lbm_group_set_alpha_work_group(self._h, int(work_group_size)))
+
+    def alpha_work_group(self) -> int:
+        w = C.c_int()
+        self._check(self._lib.lbm_group_get_alpha_work_group(self._h, C.byref(w)))
+        return w.value
+
     def autotune(self, pairs: int = 4):
         """Pick the fastest launch shape; the timed steps are simulation steps."""
         w, v, ms = C.c_int(), C.c_int(), C.c_double()
