@@ -32,7 +32,8 @@ DEFAULT_VARIANT = {"f64": 3, "f32": 3, "f16": 11, "bf16": 11}
 
 
 def build_channel(dims, storage, *, tau=0.51, cs=0.17, gx=1e-6, sphere_seed=20260101, fill=0.05, rmin=3.0,
-                  rmax=8.0, amp=0.0, seed=0, kernel_variant=0, work_group_size=256, fused_x=_capi.FUSED_X_AUTO):
+                  rmax=8.0, amp=0.0, seed=0, kernel_variant=0, work_group_size=256, fused_x=_capi.FUSED_X_AUTO,
+                  alpha_work_group=0):
     """One periodic channel in a group of one subdomain whose two x comms point
     at itself.  The periodic seam is a ghost-layer exchange; with fused_x 1 or
     _capi.FUSED_X_AUTO it runs inside the step kernel: as run tiles for one
@@ -53,6 +54,8 @@ def build_channel(dims, storage, *, tau=0.51, cs=0.17, gx=1e-6, sphere_seed=2026
     g.sync(after_beta=False)   # ghost layers start as copies of their periodic images
     g.set_exchange_mode(1)
     g.set_fused_x(fused_x)
+    if alpha_work_group:
+        g.set_alpha_work_group(alpha_work_group)
     return s, g, achieved
 
 
@@ -112,9 +115,10 @@ def main():
                          "launch shape (2: run tiles for one cell per lane, seam wavefronts for packed "
                          "pairs of a two-byte storage)")
     ap.add_argument("--wg", type=int, default=256, help="work-group size (64..512, a multiple of 64)")
+    ap.add_argument("--wg-alpha", type=int, default=0, help="alpha's work-group size (0: --wg)")
     a = ap.parse_args()
     res = run(tuple(a.dims), a.steps, tuple(a.storages), kernel_variant=a.variant, fused_x=a.fused_x,
-              work_group_size=a.wg)
+              work_group_size=a.wg, alpha_work_group=a.wg_alpha)
     print(json.dumps({"workload": f"LES channel {a.dims[0]}x{a.dims[1]}x{a.dims[2]}, periodic x, g_x 1e-6, "
                                   f"tau 0.51, Cs 0.17, 5% seeded spheres r 3-8, {a.steps} steps",
                       "results": res}))
