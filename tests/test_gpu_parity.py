@@ -313,17 +313,23 @@ def test_group_decomposed_equals_monolithic_and_minimal_exchange():
     one.close()
 
 
-def test_full_size_benchmark_decomposition_equals_monolithic():
-    """The benchmark workload at its full size (256^3 LES as 2x2x2 subdomains
-    of 128^3, kernel variant 3, x faces in the kernel, exchange mode 1, one
-    copy launch per half-step, graph replay) against a single 254^3 domain:
-    the reference's validation invariant (src/main.cpp:358-405), interior rho
-    and j bit-identical after 10 steps."""
-    say("case: full-size benchmark group vs monolithic 254^3, 10 steps")
+@pytest.mark.parametrize("alpha_wg", ["default", 0])
+def test_full_size_benchmark_decomposition_equals_monolithic(alpha_wg):
+    """The benchmark workload at its full size, launched as bench.py launches
+    it by default (256^3 LES as 2x2x2 subdomains of 128^3, kernel variant 3,
+    x faces in the kernel, exchange mode 1, one copy launch per half-step,
+    graph replay, alpha at its own work-group size), and with one size for
+    both kernels, against a single 254^3 domain: the reference's validation
+    invariant (src/main.cpp:358-405), interior rho and j bit-identical after
+    10 steps."""
     import bench
 
+    a = bench.arg_parser().parse_args([])
+    wa = a.wg_alpha if alpha_wg == "default" else alpha_wg
+    say(f"case: full-size benchmark group (wg {a.wg}, alpha {wa}) vs monolithic 254^3, 10 steps")
     w = bench.WORKLOAD
-    grp = bench.build_group(exchange_mode=1, graph=True, wg=256, kernel_variant=3, fused_x=True)
+    grp = bench.build_group(exchange_mode=a.exchange_mode, graph=True, wg=a.wg, kernel_variant=a.kernel_variant,
+                            fused_x=bool(a.fused_x), wg_alpha=wa)
     mono = dec.monolithic_size(w["domain"], w["nums"])
     one = LbmSolver(mono, (1,) * 6, inv_tau=1.0 / w["tau"], tau=w["tau"], smagorinsky_cs=w["cs"],
                     lid_velocity=w["lid"], kernel_variant=3)
