@@ -266,14 +266,18 @@ int lbm_group_get_launch(const lbm_group *g, int *work_group_size, int *kernel_v
  * per member as alpha's work-group takes (two 128-cell rows per member of a
  * two-member chain at 512) while beta keeps the group's size.  The chains
  * kept are those both sizes hold.  Results do not depend on it.  The
- * autotuner leaves it as set.  get returns the size alpha launches with. */
+ * autotuner picks it too.  get returns the size alpha launches with. */
 int lbm_group_set_alpha_work_group(lbm_group *g, int work_group_size);
 int lbm_group_get_alpha_work_group(const lbm_group *g, int *work_group_size);
 /* autotuner (new): times `pairs` step pairs for each work-group size in
  * {64,128,256,512} x kernel variant in {0,3,8,11} (1 or 2 cells per lane,
- * each with the default or the non-temporal cache policy), keeps the
- * fastest; the timed steps are real steps of the simulation (32 * (pairs + 1)
- * of them, plus one if the counter was odd) */
+ * each with the default or the non-temporal cache policy) and keeps the
+ * fastest; then, with that shape, times alpha at each other work-group size
+ * (lbm_group_set_alpha_work_group: the rows per member of its chain tiles)
+ * and keeps alpha's own size if it is faster (read it back with
+ * lbm_group_get_alpha_work_group; a size set before is replaced).  The timed
+ * steps are real steps of the simulation (38 * (pairs + 1) of them, plus one
+ * if the counter was odd) */
 int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step);
 /* the same averages split into beta and alpha step kernels */
 int lbm_group_get_timing_detail(lbm_group *g, double *beta_ms, double *alpha_ms, double *exchange_ms);

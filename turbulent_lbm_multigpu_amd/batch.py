@@ -69,7 +69,8 @@ def run_batched(cases: Sequence[Cavity], steps: int, *, autotune_pairs: int = 2,
                 keep: bool = False, colocated: bool = False, log=None):
     """All cases in one group (their buffers moved side by side into one
     allocation if `colocated`).  The autotuner times real steps of the
-    batch (16 launch shapes x (pairs + 1) step pairs); the cases are then
+    batch (16 launch shapes, then alpha at 3 work-group sizes of its own,
+    (pairs + 1) step pairs each); the cases are then
     restarted from their initial state and all `steps` run as one graph
     launch (one fused step-kernel node per half-step).  Tuning and restart
     are counted in the wall time."""
@@ -99,7 +100,8 @@ def run_batched(cases: Sequence[Cavity], steps: int, *, autotune_pairs: int = 2,
            "mlups_after_tuning": round(cells * steps / (t2 - t1) / 1e6, 1),
            "graph_launches": after["graph_launches"] - before["graph_launches"],
            "eager_half_steps": after["eager_half_steps"] - before["eager_half_steps"],
-           "launch": dict(zip(("work_group_size", "kernel_variant"), g.launch())), "autotune": tuned}
+           "launch": dict(zip(("work_group_size", "kernel_variant"), g.launch()),
+                          alpha_work_group=g.alpha_work_group()), "autotune": tuned}
     if log:
         log(f"batched: {json.dumps(res)}")
     fields = [s.moments() for s in solvers] if keep else None

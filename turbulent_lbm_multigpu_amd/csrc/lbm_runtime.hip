@@ -2235,10 +2235,12 @@ extern "C" int lbm_group_get_alpha_work_group(const lbm_group *g, int *work_grou
 // Autotuner: times `pairs` beta+alpha step pairs (with their exchanges) for
 // every work-group size in {64, 128, 256, 512} and every variant in {0, 3, 8, 11}
 // (1 or 2 cells per lane, each with the default or the non-temporal cache
-// policy) and keeps the fastest.  The steps it runs are
-// ordinary simulation steps: every launch shape computes the same
-// arithmetic, so the state is the one the same number of steps gives with
-// any shape.
+// policy) and keeps the fastest; then, with that shape for beta, times
+// alpha at each other work-group size (which sets how many rows of each
+// member an alpha chain tile holds) and keeps alpha's own size if it is
+// faster.  The steps it runs are ordinary simulation steps: every launch
+// shape computes the same arithmetic, so the state is the one the same
+// number of steps gives with any shape.
 extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step) {
     if (!g || pairs <= 0) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad autotune arguments");
     GCHK(hipSetDevice(g->device));
@@ -2253,8 +2255,6 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
     if (g->lt[0].ntiles == 0) return gfail(g, LBM_ERR_STATE, "autotune needs at least one member");
     const int wgs[4] = {64, 128, 256, 512};
     const int vars[4] = {0, 3, 8, 11};
-    double best = 1e300;
-    int bw = g->wg, bv = g->variant;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     struct Events {
         hipEvent_t *a, *b;
@@ -2265,33 +2265,57 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
     } events_guard{&e0, &e1};
     GCHK(hipEventCreate(&e0));
     GCHK(hipEventCreate(&e1));
+    // ms per step of the group's current launch shape: a warm-up pair, then
+    // `pairs` timed pairs
+    auto time_shape = [&](double *per) -> int {
+        int rc = group_prepare(g);
+        if (!rc) rc = group_launch_half(g, true, false);
+        if (!rc) rc = group_launch_half(g, false, false);
+        if (rc) return rc;
+        GCHK(hipEventRecord(e0, g->stream));
+        for (int p = 0; p < pairs && !rc; ++p) {
+            rc = group_launch_half(g, true, false);
+            if (!rc) rc = group_launch_half(g, false, false);
+        }
+        if (rc) return rc;
+        GCHK(hipEventRecord(e1, g->stream));
+        GCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        GCHK(hipEventElapsedTime(&ms, e0, e1));
+        group_advance_counters(g, g->step_counter, 2 * (pairs + 1));
+        *per = (double)ms / (2.0 * pairs);
+        return LBM_OK;
+    };
+    double best = 1e300;
+    int bw = g->wg, bv = g->variant, ba = 0;
+    int rc = lbm_group_set_alpha_work_group(g, 0);  // stage 1: one size for both kernels
+    if (rc) return rc;
     for (int wi = 0; wi < 4; ++wi)
         for (int vi = 0; vi < 4; ++vi) {
-            int rc = lbm_group_set_launch(g, wgs[wi], vars[vi] | (g->variant & 4));
-            if (!rc) rc = group_prepare(g);
+            double per = 0.0;
+            rc = lbm_group_set_launch(g, wgs[wi], vars[vi] | (g->variant & 4));
+            if (!rc) rc = time_shape(&per);
             if (rc) return rc;
-            rc = group_launch_half(g, true, false);                 // warm-up pair
-            if (!rc) rc = group_launch_half(g, false, false);
-            if (rc) return rc;
-            GCHK(hipEventRecord(e0, g->stream));
-            for (int p = 0; p < pairs && !rc; ++p) {
-                rc = group_launch_half(g, true, false);
-                if (!rc) rc = group_launch_half(g, false, false);
-            }
-            if (rc) return rc;
-            GCHK(hipEventRecord(e1, g->stream));
-            GCHK(hipEventSynchronize(e1));
-            float ms = 0.f;
-            GCHK(hipEventElapsedTime(&ms, e0, e1));
-            group_advance_counters(g, g->step_counter, 2 * (pairs + 1));
-            const double per = (double)ms / (2.0 * pairs);
             if (per < best) {
                 best = per;
                 bw = wgs[wi];
                 bv = vars[vi] | (g->variant & 4);
             }
         }
-    int rc = lbm_group_set_launch(g, bw, bv);
+    rc = lbm_group_set_launch(g, bw, bv);
+    if (rc) return rc;
+    for (int wi = 0; wi < 4; ++wi) {  // stage 2: alpha at a size of its own
+        if (wgs[wi] == bw) continue;
+        double per = 0.0;
+        rc = lbm_group_set_alpha_work_group(g, wgs[wi]);
+        if (!rc) rc = time_shape(&per);
+        if (rc) return rc;
+        if (per < best) {
+            best = per;
+            ba = wgs[wi];
+        }
+    }
+    rc = lbm_group_set_alpha_work_group(g, ba);
     if (!rc) rc = group_prepare(g);
     if (rc) return rc;
     if (best_wg) *best_wg = bw;

@@ -363,10 +363,12 @@ class LbmGroup:
         return w.value
 
     def autotune(self, pairs: int = 4):
-        """Pick the fastest launch shape; the timed steps are simulation steps."""
+        """Pick the fastest launch shape (work-group size and variant, then
+        alpha's own work-group size); the timed steps are simulation steps."""
         w, v, ms = C.c_int(), C.c_int(), C.c_double()
         self._check(self._lib.lbm_group_autotune(self._h, int(pairs), C.byref(w), C.byref(v), C.byref(ms)))
-        return {"work_group_size": w.value, "kernel_variant": v.value, "ms_per_step": ms.value}
+        return {"work_group_size": w.value, "kernel_variant": v.value, "alpha_work_group": self.alpha_work_group(),
+                "ms_per_step": ms.value}
 
     def wait(self):
         self._check(self._lib.lbm_group_wait(self._h))
