@@ -1,4 +1,5 @@
-"""Seeded random single domains against the CPU oracle, bit for bit.
+"""Seeded random single domains and decompositions against the CPU oracle,
+bit for bit.
 
 Each case draws a domain size, the six face flags (obstacle, fluid, velocity
 injection or ghost: the linear index wrap of the reference then routes the
@@ -62,3 +63,43 @@ def test_random_domain_equals_oracle(case):
     np.testing.assert_array_equal(s.store(_capi.FIELD_VELOCITY).reshape(3, -1), o.vel)
     np.testing.assert_array_equal(s.store(_capi.FIELD_DENSITY).ravel(), o.dens)
     s.close()
+
+
+NUMS = [(2, 1, 1), (3, 1, 1), (1, 2, 1), (1, 1, 2), (2, 2, 1), (2, 1, 2), (1, 2, 2), (2, 2, 2)]
+
+
+@pytest.mark.parametrize("case", range(24))
+def test_random_decomposition_equals_oracle(case):
+    """A seeded random decomposition (subdomain sizes, periodic axes, rings of
+    one and two included) with the reference's exchange (all 19 populations
+    after alpha, the n.e > 0 ones after beta, x then y then z) and a random
+    launch shape, against the oracle's decomposition stepped the same way:
+    every slot of every subdomain bit for bit after odd and even counts."""
+    from tests.test_gpu_parity import build_gpu_group
+    from tests.test_oracle import build_oracle_decomposition
+
+    rng = np.random.default_rng(9_000 + case)
+    nums = NUMS[int(rng.integers(len(NUMS)))]
+    if rng.random() < 0.25:
+        nums = (1,) + nums[1:]   # an x ring of one when periodic
+    dims = tuple(nums[a] * int(rng.integers(6, 22)) for a in range(3))
+    periodic = tuple(bool(rng.random() < 0.4) for _ in range(3))
+    kw = dict(inv_tau=1 / 0.55, tau=0.55, smagorinsky_cs=(0.0, 0.17)[int(rng.integers(2))])
+    wg, variant = (64, 128, 256, 512)[int(rng.integers(4))], VARIANTS[int(rng.integers(len(VARIANTS)))]
+    alpha_wg, fused = (0, 128, 512)[int(rng.integers(3))], (0, 1)[int(rng.integers(2))]
+    say(f"case {case}: dims {dims} nums {nums} periodic {periodic} cs {kw['smagorinsky_cs']} wg {wg} "
+        f"variant {variant} alpha {alpha_wg} fused {fused}")
+    multi, _ = build_oracle_decomposition(dims, nums, lid=0.06, amp=1e-3, seed=case, periodic=periodic, **kw)
+    grp, _, _ = build_gpu_group(dims, nums, lid=0.06, amp=1e-3, seed=case, mode=0, periodic=periodic, **kw)
+    grp.set_launch(wg, variant)
+    grp.set_alpha_work_group(alpha_wg)
+    grp.set_fused_x(fused)
+    for steps in (3, 4):
+        multi.run(steps)
+        grp.step(steps)
+        grp.wait()
+        for k, (sub, s) in enumerate(zip(multi.subs, grp.solvers)):
+            np.testing.assert_array_equal(gpu_dd(s), sub["domain"].dd, err_msg=f"member {k}")
+    grp.close()
+    for s in grp.solvers:
+        s.close()

@@ -401,11 +401,11 @@ def test_region_store_set_and_norm_filter():
 
 
 # ------------------------------------------------- decomposition (validation)
-def build_oracle_decomposition(domain_size, nums, *, lid, amp, seed, **kw):
+def build_oracle_decomposition(domain_size, nums, *, lid, amp, seed, periodic=(False, False, False), **kw):
     from turbulent_lbm_multigpu_amd import decomposition as dec
     subs = []
     mono = dec.monolithic_size(domain_size, nums)
-    for sd in dec.decompose(domain_size, nums):
+    for sd in dec.decompose(domain_size, nums, periodic=periodic):
         bc = [sd.bc[0][0], sd.bc[0][1], sd.bc[1][0], sd.bc[1][1], sd.bc[2][0], sd.bc[2][1]]
         d = orc.OracleDomain(sd.domain.size, bc, lid_velocity=lid, **kw)
         d.reset(amplitude=amp, seed=seed, key_origin=dec.key_origin(sd), key_dims=mono)
