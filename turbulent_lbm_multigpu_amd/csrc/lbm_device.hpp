@@ -1179,4 +1179,39 @@ __global__ void __launch_bounds__(512) step_kernel(const DomDev *__restrict__ do
         if (g0 + j < cells) cell_scalar<SP, BETA, NTS, NTL>(pl, (unsigned)(g0 + j), cells, d, wrap, D, run);
 }
 
+// Launch of step_kernel for storage ST: the half-step and the variant bits
+// become template arguments; bit 2 of the variant turns the XCD-contiguous
+// tile map off; `lds` reserves LDS per work-group to cap residency.  The 4-
+// and 8-byte storages are instantiated in lbm_runtime.hip, the two-byte ones
+// in lbm_step_2byte.hip (a translation unit of their own: see there).
+template <int ST>
+hipError_t launch_step(bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs, const int *tile_table,
+                       hipStream_t stream, long long *ctr, const XChain *chains) {
+    const int xcd = (variant & 4) ? 0 : 1;
+    auto go = [&](auto b, auto vv) {
+        constexpr bool B = decltype(b)::value;
+        constexpr int V = decltype(vv)::value;
+        step_kernel<ST, B, V><<<tiles, wg, lds, stream>>>(descs, tile_table, chains, xcd, ctr);
+    };
+    auto pick = [&](auto b) {
+        switch (variant & ~4) {
+            case 0: go(b, std::integral_constant<int, 0>()); break;
+            case 1: go(b, std::integral_constant<int, 1>()); break;
+            case 2: go(b, std::integral_constant<int, 2>()); break;
+            case 3: go(b, std::integral_constant<int, 3>()); break;
+            case 8: go(b, std::integral_constant<int, 8>()); break;
+            case 11: go(b, std::integral_constant<int, 11>()); break;
+            case 16: go(b, std::integral_constant<int, 16>()); break;
+            default: go(b, std::integral_constant<int, 0>()); break;
+        }
+    };
+    if (beta) pick(std::true_type());
+    else pick(std::false_type());
+    return hipGetLastError();
+}
+
+// launch_step for the FP16 / BF16 storages (2, 3), defined in lbm_step_2byte.hip
+hipError_t launch_step_2byte(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
+                             const int *tile_table, hipStream_t stream, long long *ctr, const XChain *chains);
+
 }  // namespace lbm

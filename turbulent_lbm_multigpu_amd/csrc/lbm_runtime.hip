@@ -514,38 +514,20 @@ int upload_desc(lbm_solver *s) {
     return LBM_OK;
 }
 
-// Launch of the step kernel: storage type, half-step and variant bits become
-// template arguments; bit 2 of the variant turns the XCD-contiguous tile map
-// off; lds_per_block reserves LDS to cap the number of resident work-groups.
+// Launch of the step kernel (launch_step in lbm_device.hpp): the FP32 and
+// FP64 kernels are instantiated here, the two-byte storages' in
+// lbm_step_2byte.hip.
 hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
                               const int *tile_table, hipStream_t stream, long long *ctr = nullptr,
                               const XChain *chains = nullptr) {
-    const int xcd = (variant & 4) ? 0 : 1;
-    const int v = variant & ~4;
-    dispatch_storage(storage, [&](auto st) -> int {
-        constexpr int ST = decltype(st)::value;
-        auto go = [&](auto b, auto vv) {
-            constexpr bool B = decltype(b)::value;
-            constexpr int V = decltype(vv)::value;
-            step_kernel<ST, B, V><<<tiles, wg, lds, stream>>>(descs, tile_table, chains, xcd, ctr);
-        };
-        auto pick = [&](auto b) {
-            switch (v) {
-                case 0: go(b, std::integral_constant<int, 0>()); break;
-                case 1: go(b, std::integral_constant<int, 1>()); break;
-                case 2: go(b, std::integral_constant<int, 2>()); break;
-                case 3: go(b, std::integral_constant<int, 3>()); break;
-                case 8: go(b, std::integral_constant<int, 8>()); break;
-                case 11: go(b, std::integral_constant<int, 11>()); break;
-                case 16: go(b, std::integral_constant<int, 16>()); break;
-                default: go(b, std::integral_constant<int, 0>()); break;
-            }
-        };
-        if (beta) pick(std::true_type());
-        else pick(std::false_type());
-        return 0;
-    });
-    return hipGetLastError();
+    switch (storage) {
+        case LBM_STORAGE_F32:
+            return launch_step<0>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
+        case LBM_STORAGE_F64:
+            return launch_step<1>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
+        default:
+            return launch_step_2byte(storage, beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
+    }
 }
 
 int launch_step(lbm_solver *s, bool beta) {
