@@ -778,6 +778,77 @@ __device__ __forceinline__ void pair_st(gchar *p, unsigned s, unsigned v) {
     raw_st<NT, unsigned short>(p, cell_base(s + 1u) * 2u, (unsigned short)(v >> 16));
 }
 
+// pair_ld / pair_st through the linear wrap W where the run nears a domain end
+__device__ __forceinline__ unsigned pair_ld_w(const gchar *p, int s, int cells, bool wrap) {
+    if (!wrap) return pair_ld(p, (unsigned)s);
+    return (unsigned)raw_ld<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s, cells)) * 2u) |
+           ((unsigned)raw_ld<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s + 1, cells)) * 2u) << 16);
+}
+__device__ __forceinline__ void pair_st_w(gchar *p, int s, int cells, bool wrap, bool nt, unsigned v) {
+    if (!wrap) {
+        if (nt) pair_st<true>(p, (unsigned)s, v);
+        else pair_st<false>(p, (unsigned)s, v);
+        return;
+    }
+    raw_st<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s, cells)) * 2u, (unsigned short)v);
+    raw_st<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s + 1, cells)) * 2u, (unsigned short)(v >> 16));
+}
+
+// Pair runs: a wavefront whose 64 lanes hold the packed pairs of the
+// consecutive cells c0 .. c0+127 (c0 uniform).  Where a shifted run
+// c0 - d_i (gather) or c0 + d_i (scatter) lies in one layout block, lane l's
+// pair of it is the run's wavefront-uniform base plus 4 l bytes: the address
+// costs scalar instructions only, where the per-lane form spends about six
+// vector instructions per access (index, block split, straddle check), for
+// each of beta's 36 shifted accesses.  The two-byte kernels issue vector
+// instructions near the SIMDs' limit, so these count.
+__device__ __forceinline__ bool pair_run_in_block(unsigned c0) { return (c0 & (kBlk - 1u)) <= kBlk - 128u; }
+
+// every gather run (ld) / every scatter run (st) of the wavefront in one block
+__device__ __forceinline__ void pair_runs(unsigned c0, const int *d, bool &ld, bool &st) {
+    ld = st = true;
+#pragma unroll
+    for (int i = 0; i < 18; ++i) {
+        ld = ld && pair_run_in_block(c0 - (unsigned)d[i]);
+        st = st && pair_run_in_block(c0 + (unsigned)d[i]);
+    }
+}
+
+// beta's 18 gathered pairs (slot opp(i) at g - d_i) and the own slot 18, per
+// lane (pair_ld) or, with `run`, from the runs' uniform bases (lo = 4 l)
+__device__ __forceinline__ void pair_gather(gchar *const *pl, unsigned g, unsigned c0, unsigned lo, bool run,
+                                            const int *d, int cells, bool wrap, unsigned *P) {
+    if (run) {
+#pragma unroll
+        for (int i = 0; i < 18; ++i)
+            P[i] = *(const gpair *)(pl[opp(i)] + (size_t)cell_base(c0 - (unsigned)d[i]) * 2u + lo);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 18; ++i) P[i] = pair_ld_w(pl[opp(i)], (int)g - d[i], cells, wrap);
+    }
+    P[18] = raw_ld<false, unsigned>(pl[18], cell_base(g) * 2u);
+}
+
+// beta's 18 scattered pairs (slot q at g + d_q) and the own slot 18; the
+// non-temporal policy only for the e.x = 0 slots at 4-byte alignment
+template <bool NTS>
+__device__ __forceinline__ void pair_scatter(gchar *const *pl, unsigned g, unsigned c0, unsigned lo, bool run,
+                                             const int *d, int cells, bool wrap, const unsigned *P) {
+    if (run) {
+#pragma unroll
+        for (int q = 0; q < 18; ++q) {
+            const unsigned s0 = c0 + (unsigned)d[q];
+            gchar *b = pl[q] + (size_t)cell_base(s0) * 2u;
+            if (NTS && ex(q) == 0 && (s0 & 1u) == 0u) raw_st<true, unsigned>(b, lo, P[q]);  // uniform
+            else *(gpair *)(b + lo) = P[q];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 18; ++q) pair_st_w(pl[q], (int)g + d[q], cells, wrap, NTS && ex(q) == 0, P[q]);
+    }
+    raw_st<NTS, unsigned>(pl[18], cell_base(g) * 2u, P[18]);
+}
+
 // Two x-consecutive cells g0, g0+1 per work-item in a two-byte storage (g0
 // even, away from the domain ends): every population access moves one 32-bit
 // word holding both cells, and the words stay packed in registers.  The two
@@ -790,12 +861,20 @@ __device__ __forceinline__ void cells_pair2(gchar *const *pl, unsigned g0, int c
     static_assert(SP::kBytes == 2, "packed pairs hold two-byte storages");
     using T = typename SP::T;
     const unsigned fl = raw_ld<false, unsigned short>((const gchar *)D->flags, g0);
-    const unsigned own = cell_base(g0) * 2u;
     unsigned P[19];
+    // beta: pair runs when all 64 lanes are here (consecutive pairs from the
+    // lane-0 cell c0; lanes at the domain end take another path)
+    unsigned c0 = g0, lo = 0u;
+    bool rl = false, rs = false;
+    if (BETA) {
+        c0 = __builtin_amdgcn_readfirstlane(g0);
+        lo = (g0 - c0) * 2u;
+        if (__ballot(1) == ~0ull) pair_runs(c0, d, rl, rs);
+        pair_gather(pl, g0, c0, lo, rl, d, cells, false, P);
+    } else {
+        const unsigned own = cell_base(g0) * 2u;
 #pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        if (BETA && i < 18) P[i] = pair_ld(pl[opp(i)], g0 - (unsigned)d[i]);
-        else P[i] = raw_ld<NTL && !BETA, unsigned>(pl[i], own);
+        for (int i = 0; i < 19; ++i) P[i] = raw_ld<NTL, unsigned>(pl[i], own);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -821,15 +900,10 @@ __device__ __forceinline__ void cells_pair2(gchar *const *pl, unsigned g0, int c
     // addresses live through both collisions (~40 VGPRs).
     unsigned gs = g0;
     asm volatile("" : "+v"(gs));
-    const unsigned own_st = cell_base(gs) * 2u;
     if (BETA) {
-#pragma unroll
-        for (int i = 0; i < 18; ++i) {
-            if (NTS && ex(i) == 0) pair_st<true>(pl[i], gs + (unsigned)d[i], P[i]);
-            else pair_st<false>(pl[i], gs + (unsigned)d[i], P[i]);
-        }
-        raw_st<NTS, unsigned>(pl[18], own_st, P[18]);
+        pair_scatter<NTS>(pl, gs, c0, lo, rs, d, cells, false, P);
     } else {
+        const unsigned own_st = cell_base(gs) * 2u;
 #pragma unroll
         for (int k = 0; k < 19; ++k) raw_st<NTS, unsigned>(pl[k], own_st, P[k]);
     }
@@ -1010,21 +1084,6 @@ __device__ __forceinline__ void chain_tile(const DomDev *__restrict__ doms, cons
 // take what the copy exchange would give them: after alpha the slots of
 // their receiver masks, in beta the populations they scatter into the inner
 // layer (e.x = +1 at x = 0, e.x = -1 at x = X-1).
-__device__ __forceinline__ unsigned pair_ld_w(const gchar *p, int s, int cells, bool wrap) {
-    if (!wrap) return pair_ld(p, (unsigned)s);
-    return (unsigned)raw_ld<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s, cells)) * 2u) |
-           ((unsigned)raw_ld<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s + 1, cells)) * 2u) << 16);
-}
-__device__ __forceinline__ void pair_st_w(gchar *p, int s, int cells, bool wrap, bool nt, unsigned v) {
-    if (!wrap) {
-        if (nt) pair_st<true>(p, (unsigned)s, v);
-        else pair_st<false>(p, (unsigned)s, v);
-        return;
-    }
-    raw_st<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s, cells)) * 2u, (unsigned short)v);
-    raw_st<false, unsigned short>(p, cell_base((unsigned)wrap_idx(s + 1, cells)) * 2u, (unsigned short)(v >> 16));
-}
-
 template <class SP, bool BETA, bool NTS, bool NTL>
 __device__ __forceinline__ void seam_pairs(const DomDev *__restrict__ D, int tile) {
     using T = typename SP::T;
@@ -1057,12 +1116,20 @@ __device__ __forceinline__ void seam_pairs(const DomDev *__restrict__ D, int til
     }
     // the pair's loads and collisions (as cells_pair2, with the linear wrap per half)
     const unsigned fl = raw_ld<false, unsigned short>((const gchar *)D->flags, (unsigned)g);
-    const unsigned own = cell_base((unsigned)g) * 2u;
     unsigned P[19];
+    // beta: pair runs for the wavefronts of consecutive pairs (not the split
+    // first one), all 64 lanes present, away from the domain ends
+    unsigned c0 = (unsigned)g, lo = 0u;
+    bool rl = false, rs = false;
+    if (BETA) {
+        c0 = __builtin_amdgcn_readfirstlane((unsigned)g);
+        lo = ((unsigned)g - c0) * 2u;
+        if (k != 0 && !wrap && __ballot(1) == ~0ull) pair_runs(c0, d, rl, rs);
+        pair_gather(pl, (unsigned)g, c0, lo, rl, d, cells, wrap, P);
+    } else {
+        const unsigned own = cell_base((unsigned)g) * 2u;
 #pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        if (BETA && i < 18) P[i] = pair_ld_w(pl[opp(i)], g - d[i], cells, wrap);
-        else P[i] = raw_ld<NTL && !BETA, unsigned>(pl[i], own);
+        for (int i = 0; i < 19; ++i) P[i] = raw_ld<NTL, unsigned>(pl[i], own);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -1094,11 +1161,12 @@ __device__ __forceinline__ void seam_pairs(const DomDev *__restrict__ D, int til
     // store addresses from a fresh copy of the cell index (see cells_pair2)
     int gs = g;
     asm volatile("" : "+v"(gs));
-    const unsigned own_st = cell_base((unsigned)gs) * 2u;
+    if (BETA) {
+        pair_scatter<NTS>(pl, (unsigned)gs, c0, lo, rs, d, cells, wrap, P);
+    } else {
+        const unsigned own_st = cell_base((unsigned)gs) * 2u;
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-        if (BETA && q < 18) pair_st_w(pl[q], gs + d[q], cells, wrap, NTS && ex(q) == 0, P[q]);
-        else raw_st<NTS, unsigned>(pl[q], own_st, P[q]);
+        for (int q = 0; q < 19; ++q) raw_st<NTS, unsigned>(pl[q], own_st, P[q]);
     }
 }
 

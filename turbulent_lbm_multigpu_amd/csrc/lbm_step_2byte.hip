@@ -4,8 +4,8 @@
 // They are compiled with floating-point contraction: the compiler may fuse a
 // product and a sum into one FMA across statements.  These kernels move half
 // the bytes of FP32 per cell with about the same per-cell instruction count,
-// so their vector-instruction issue, not HBM, bounds them, and the fused
-// collision runs 2-3 % faster.  The FP32 and FP64 kernels (lbm_runtime.hip)
+// so their vector-instruction issue runs near the SIMDs' limit, and the
+// fused collision (9-12 % fewer vector instructions) runs 2-3 % faster.  The FP32 and FP64 kernels (lbm_runtime.hip)
 // keep every rounding of the reference's expressions (no contraction): their
 // bit-for-bit parity with the CPU oracle depends on it.  The two-byte
 // storages have no such counterpart; their results are bounded against the
