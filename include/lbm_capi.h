@@ -45,7 +45,10 @@ extern "C" {
 #define LBM_FLAG_VELOCITY_INJECTION 4
 #define LBM_FLAG_GHOST_LAYER 8
 
-/* storage of the distributions (compute type: f32 except LBM_STORAGE_F64) */
+/* storage of the distributions (compute type: f32 except LBM_STORAGE_F64).
+ * FP32 and FP64 evaluate the reference's expressions rounding by rounding
+ * (no fused multiply-adds); the two-byte storages' step kernels fuse them
+ * (FP contraction), which changes their last bits, not their error bounds. */
 #define LBM_STORAGE_F32 0
 #define LBM_STORAGE_F64 1
 #define LBM_STORAGE_F16_SHIFTED 2  /* stores f_i - w_i as IEEE half */
