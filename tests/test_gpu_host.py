@@ -24,11 +24,13 @@ def run(*args, cwd=None, timeout=120):
     return subprocess.run([EXE, *args], capture_output=True, text=True, timeout=timeout, cwd=cwd)
 
 
-@pytest.mark.parametrize("fused_x", [0, 1])
-def test_validation_mode_all_subdomains_match(tmp_path, fused_x):
-    # x faces inside the step kernel (the default) or by the copy launch: the same cells
+@pytest.mark.parametrize("fused_x,alpha_wg", [(0, 0), (1, 0), (1, 512)])
+def test_validation_mode_all_subdomains_match(tmp_path, fused_x, alpha_wg):
+    # x faces inside the step kernel (the default) or by the copy launch, alpha at the group's
+    # work-group size or at its own: the same cells
     text = open(os.path.join(DATA, "conf_validate_les.xml")).read().replace(
-        "<exchange-mode>1</exchange-mode>", f"<exchange-mode>1</exchange-mode><fused-x>{fused_x}</fused-x>")
+        "<exchange-mode>1</exchange-mode>",
+        f"<exchange-mode>1</exchange-mode><fused-x>{fused_x}</fused-x><alpha-work-group>{alpha_wg}</alpha-work-group>")
     conf = str(tmp_path / "conf_fused.xml")
     open(conf, "w").write(text)
     r = run("-c", conf, cwd=tmp_path)

@@ -6,7 +6,8 @@
 //   <lattice> <tau/> <lid-velocity/> <body-force><x/><y/><z/></body-force>
 //             <smagorinsky-cs/> <storage>f32|f64|f16|bf16</storage>
 //             <periodic><x/><y/><z/></periodic> <forcing>reference|body</forcing>
-//             <kernel-variant/> <exchange-mode/> <fused-x/> <perturbation><amplitude/><seed/></perturbation>
+//             <kernel-variant/> <exchange-mode/> <fused-x/> <alpha-work-group/>
+//             <perturbation><amplitude/><seed/></perturbation>
 //   </lattice>
 #pragma once
 
@@ -144,8 +145,11 @@ public:
     unsigned long long perturbation_seed = 0;
     int exchange_mode = 0;  // 0: all 19 populations after alpha (reference), 1: the 5 an inner cell reads
     // x faces: 1 inside the step kernel, 0 the copy launch (same results), 2 (LBM_FUSED_X_AUTO) the
-    // faster of the two for the storage: inside the kernel for f32/f64, the copy launch for f16/bf16
+    // faster of the two for the storage: inside the kernel for f32/f64; for f16/bf16 only the seam
+    // wavefronts of a periodic ring of one, the copy launch elsewhere
     int fused_x = LBM_FUSED_X_AUTO;
+    // alpha's own work-group size (lbm_group_set_alpha_work_group; 0: <kernel-count>)
+    int alpha_work_group = 0;
 
     CConfiguration() {}
     explicit CConfiguration(const std::string &file_name) { loadFile(file_name); }
@@ -229,6 +233,7 @@ private:
             if (auto *c = lat->child("kernel-variant")) lattice.kernel_variant = atoi(c->text.c_str());
             if (auto *c = lat->child("exchange-mode")) exchange_mode = atoi(c->text.c_str());
             if (auto *c = lat->child("fused-x")) fused_x = atoi(c->text.c_str());
+            if (auto *c = lat->child("alpha-work-group")) alpha_work_group = atoi(c->text.c_str());
             if (auto *c = lat->child("storage")) {
                 const std::string &v = c->text;
                 lattice.storage = v == "f64" ? LBM_STORAGE_F64 : v == "f16" ? LBM_STORAGE_F16_SHIFTED

@@ -151,6 +151,8 @@ public:
             throw "CManager: <exchange-mode> must be 0 or 1";
         if (lbm_group_set_fused_x(_group, cfg->fused_x) != LBM_OK)
             throw "CManager: <fused-x> must be 0, 1 or 2";
+        if (lbm_group_set_alpha_work_group(_group, cfg->alpha_work_group) != LBM_OK)
+            throw "CManager: <alpha-work-group> must be 0 or 64..512, a multiple of 64";
         // the first controller drives the group; stepping through any other throws
         for (size_t i = 0; i < _controllers.size(); ++i) _controllers[i]->attachGroup(_group, i == 0);
     }
