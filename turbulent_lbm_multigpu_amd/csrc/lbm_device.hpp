@@ -1253,7 +1253,7 @@ __global__ void __launch_bounds__(512) step_kernel(const DomDev *__restrict__ do
 // and 8-byte storages are instantiated in lbm_runtime.hip, the two-byte ones
 // in lbm_step_2byte.hip (a translation unit of their own: see there).
 template <int ST>
-hipError_t launch_step(bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs, const int *tile_table,
+hipError_t launch_step_storage(bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs, const int *tile_table,
                        hipStream_t stream, long long *ctr, const XChain *chains) {
     const int xcd = (variant & 4) ? 0 : 1;
     auto go = [&](auto b, auto vv) {
@@ -1278,7 +1278,7 @@ hipError_t launch_step(bool beta, int variant, int lds, int tiles, int wg, const
     return hipGetLastError();
 }
 
-// launch_step for the FP16 / BF16 storages (2, 3), defined in lbm_step_2byte.hip
+// launch_step_storage for the FP16 / BF16 storages (2, 3), defined in lbm_step_2byte.hip
 hipError_t launch_step_2byte(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
                              const int *tile_table, hipStream_t stream, long long *ctr, const XChain *chains);
 

@@ -514,7 +514,7 @@ int upload_desc(lbm_solver *s) {
     return LBM_OK;
 }
 
-// Launch of the step kernel (launch_step in lbm_device.hpp): the FP32 and
+// Launch of the step kernel (launch_step_storage in lbm_device.hpp): the FP32 and
 // FP64 kernels are instantiated here, the two-byte storages' in
 // lbm_step_2byte.hip.
 hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
@@ -522,9 +522,9 @@ hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int 
                               const XChain *chains = nullptr) {
     switch (storage) {
         case LBM_STORAGE_F32:
-            return launch_step<0>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
+            return launch_step_storage<0>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
         case LBM_STORAGE_F64:
-            return launch_step<1>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
+            return launch_step_storage<1>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
         default:
             return launch_step_2byte(storage, beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
     }
@@ -1463,9 +1463,6 @@ static bool group_pairs(const lbm_group *g) {
     return elem_size(g->storage) == 2 && cells_per_lane(g->variant & ~4) == 2;
 }
 
-// Descriptor and tile tables of the fused launch for the group's current
-// members and launch shape (work-group size and cells per lane); also
-// re-publishes the host half-step count to the device counter.
 // Fills the launch tables of the step kernel at work-group size wg: every
 // member's descriptor with its tile range, and the tile map (member index,
 // or -chain-1 for the tiles of a chain).
@@ -1564,6 +1561,9 @@ static int build_tables(lbm_group *g, int wg, LaunchTables &T) {
 
 static bool alpha_split(const lbm_group *g) { return g->wg_alpha != 0 && g->wg_alpha != g->wg; }
 
+// Descriptor and tile tables of the fused launches (beta's, and alpha's at
+// its own work-group size) for the group's current members and launch shape;
+// also re-publishes the host half-step count to the device counter.
 static int group_upload_descs(lbm_group *g) {
     int rc = build_tables(g, g->wg, g->lt[0]);
     if (!rc && alpha_split(g)) rc = build_tables(g, g->wg_alpha, g->lt[1]);
