@@ -70,12 +70,14 @@ def batching_leg(log):
     cases = batch.make_cavities(512, 20260101)
     b, _ = batch.run_batched(cases, 1000, autotune_pairs=2)
     s, _ = batch.run_sequential(cases, 1000)
-    log(f"[bench] batching: {b['cases_per_s']} cases/s batched ({b['graph_launches']} graph launch for "
-        f"{b['steps']} steps), {s['cases_per_s']} one at a time")
+    log(f"[bench] batching: {b['cases_per_s']} cases/s batched ({b['tuning_steps_kept']} steps while tuning, "
+        f"{b['graph_launches']} graph launch for the other {b['steps'] - b['tuning_steps_kept']}), "
+        f"{s['cases_per_s']} one at a time")
     return {"workload": "512 LES cavities, edges U{16..96}, lid U[0.02,0.1], tau lognormal(0.52,0.01) >= 0.5002, "
                         "Cs 0.17, FP32, 1000 steps, seed 20260101",
             "cases_per_s": b["cases_per_s"], "mlups": b["mlups"], "mlups_after_tuning": b["mlups_after_tuning"],
             "wall_s": b["wall_s"], "tune_s": b["tune_s"], "run_s": b["run_s"],
+            "tuning_steps_kept": b["tuning_steps_kept"],
             "graph_launches": b["graph_launches"], "eager_half_steps": b["eager_half_steps"],
             "launch": b["launch"], "one_at_a_time_cases_per_s": s["cases_per_s"],
             "speedup": round(b["cases_per_s"] / s["cases_per_s"], 3)}

@@ -130,6 +130,22 @@ def test_batched_cavities_equal_sequential():
         np.testing.assert_array_equal(j1, j2)
 
 
+def test_batched_keeps_the_tuners_steps():
+    """The autotuner's steps are simulation steps of every case: with more
+    steps asked for than tuning takes, the batch keeps them and runs the
+    rest in one graph launch, bit-identical to the cases run one at a time."""
+    say("case: 10 cavities, 120 steps, tuning steps kept")
+    cases = batch.make_cavities(10, seed=9, edge=(16, 28))
+    rb, fb = batch.run_batched(cases, 120, autotune_pairs=1, keep=True)
+    rs, fs = batch.run_sequential(cases, 120, keep=True)
+    say(f"  kept {rb['tuning_steps_kept']} tuning steps, launch {rb['launch']}")
+    assert 0 < rb["tuning_steps_kept"] < 120 and rb["tuning_steps_kept"] % 2 == 0
+    assert rb["graph_launches"] == 1 and rb["eager_half_steps"] == 0
+    for (r1, j1), (r2, j2) in zip(fb, fs):
+        np.testing.assert_array_equal(r1, r2)
+        np.testing.assert_array_equal(j1, j2)
+
+
 def test_full_size_batch_equals_sequential():
     """The batching workload at its full size (the benchmark's 512 seeded
     cavities, edges 16..96): batched with the autotuner's shapes and one at a
