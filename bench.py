@@ -278,7 +278,8 @@ def arg_parser() -> argparse.ArgumentParser:
     ap.add_argument("--wg", type=int, default=256)
     ap.add_argument("--wg-alpha", type=int, default=512,
                     help="alpha's work-group size (0: --wg); at 512 its chain tiles hold two rows per subdomain")
-    ap.add_argument("--kernel-variant", type=int, default=3)
+    ap.add_argument("--kernel-variant", type=int, default=7,
+                    help="non-temporal loads and stores (3), without the XCD tile map (| 4)")
     ap.add_argument("--lds-per-block", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--graph-pairs", type=int, default=1000,

@@ -278,8 +278,9 @@ int lbm_group_get_alpha_work_group(const lbm_group *g, int *work_group_size);
  * fastest; then, with that shape, times alpha at each other work-group size
  * (lbm_group_set_alpha_work_group: the rows per member of its chain tiles)
  * and keeps alpha's own size if it is faster (read it back with
- * lbm_group_get_alpha_work_group; a size set before is replaced).  The timed
- * steps are real steps of the simulation (38 * (pairs + 1) of them, plus one
+ * lbm_group_get_alpha_work_group; a size set before is replaced); last, the
+ * chosen shape with the XCD tile map (variant bit 4) switched.  The timed
+ * steps are real steps of the simulation (40 * (pairs + 1) of them, plus one
  * if the counter was odd) */
 int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step);
 /* the same averages split into beta and alpha step kernels */

@@ -122,7 +122,7 @@ def test_batched_cavities_equal_sequential():
     rb, fb = batch.run_batched(cases, 60, autotune_pairs=1, keep=True)
     rs, fs = batch.run_sequential(cases, 60, keep=True)
     say(f"  batched {rb['launch']}, sequential {rs['cases_per_s']} cases/s")
-    assert rb["autotune"] is not None  # the tuner's 16 + 3 shapes ran, then the cases restarted
+    assert rb["autotune"] is not None  # the tuner's 16 + 3 + 1 shapes ran, then the cases restarted
     assert rb["launch"]["alpha_work_group"] == rb["autotune"]["alpha_work_group"] in (64, 128, 256, 512)
     assert rb["graph_launches"] == 1 and rb["eager_half_steps"] == 0   # 60 steps in one graph launch
     for (r1, j1), (r2, j2) in zip(fb, fs):

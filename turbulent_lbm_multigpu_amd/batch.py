@@ -70,7 +70,7 @@ def run_batched(cases: Sequence[Cavity], steps: int, *, autotune_pairs: int = 2,
     """All cases in one group (their buffers moved side by side into one
     allocation if `colocated`).  The autotuner times real steps of the
     batch (16 launch shapes, then alpha at 3 work-group sizes of its own,
-    (pairs + 1) step pairs each): every launch shape computes the same
+    then the XCD tile map switched, (pairs + 1) step pairs each): every launch shape computes the same
     arithmetic, so those are `steps` of every case like any other, and the
     rest run as one graph launch (one fused step-kernel node per
     half-step).  If tuning took more steps than `steps`, the cases restart

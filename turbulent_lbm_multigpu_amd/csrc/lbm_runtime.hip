@@ -2220,9 +2220,11 @@ extern "C" int lbm_group_get_alpha_work_group(const lbm_group *g, int *work_grou
 // policy) and keeps the fastest; then, with that shape for beta, times
 // alpha at each other work-group size (which sets how many rows of each
 // member an alpha chain tile holds) and keeps alpha's own size if it is
-// faster.  The steps it runs are ordinary simulation steps: every launch
-// shape computes the same arithmetic, so the state is the one the same
-// number of steps gives with any shape.
+// faster; last, it times the chosen shape with the XCD tile map switched
+// (variant bit 4) and keeps that if it is faster.  The steps it runs are
+// ordinary simulation steps: every launch shape computes the same
+// arithmetic, so the state is the one the same number of steps gives with
+// any shape.
 extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step) {
     if (!g || pairs <= 0) return record(nullptr, LBM_ERR_INVALID_ARGUMENT, "bad autotune arguments");
     GCHK(hipSetDevice(g->device));
@@ -2298,6 +2300,18 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
         }
     }
     rc = lbm_group_set_alpha_work_group(g, ba);
+    if (rc) return rc;
+    {  // stage 3: the XCD tile map the other way round
+        double per = 0.0;
+        rc = lbm_group_set_launch(g, bw, bv ^ 4);
+        if (!rc) rc = time_shape(&per);
+        if (rc) return rc;
+        if (per < best) {
+            best = per;
+            bv ^= 4;
+        }
+    }
+    rc = lbm_group_set_launch(g, bw, bv);
     if (!rc) rc = group_prepare(g);
     if (rc) return rc;
     if (best_wg) *best_wg = bw;

@@ -364,7 +364,8 @@ class LbmGroup:
 
     def autotune(self, pairs: int = 4):
         """Pick the fastest launch shape (work-group size and variant, then
-        alpha's own work-group size); the timed steps are simulation steps."""
+        alpha's own work-group size, then the XCD tile map on or off); the
+        timed steps are simulation steps."""
         w, v, ms = C.c_int(), C.c_int(), C.c_double()
         self._check(self._lib.lbm_group_autotune(self._h, int(pairs), C.byref(w), C.byref(v), C.byref(ms)))
         return {"work_group_size": w.value, "kernel_variant": v.value, "alpha_work_group": self.alpha_work_group(),
