@@ -329,10 +329,8 @@ __device__ __forceinline__ T vel2_of(T vx, T vy, T vz) {
 // tau_eff = (tau0 + sqrt(tau0^2 + les_coeff*|Pi_neq|/rho)) / 2,
 // les_coeff = 18 sqrt(2) Cs^2, Pi_neq_ab = sum e_a e_b f - rho/3 d_ab - j_a j_b.
 // The 19 populations are in this work-item's registers: no cross-lane traffic.
-// rho_m: the density as summed from f[] (rho, or rho - 1 for shifted f),
-// rho: the full density.
 template <typename T>
-__device__ __forceinline__ T les_inv_tau(const T *f, T rho_m, T rho, T jx, T jy, T jz, T tau0, T les_coeff) {
+__device__ __forceinline__ T les_inv_tau(const T *f, T rho, T jx, T jy, T jz, T tau0, T les_coeff) {
     T pxx = f[0] + f[1];
     pxx = pxx + f[4]; pxx = pxx + f[5]; pxx = pxx + f[6]; pxx = pxx + f[7];
     pxx = pxx + f[8]; pxx = pxx + f[9]; pxx = pxx + f[10]; pxx = pxx + f[11];
@@ -345,7 +343,7 @@ __device__ __forceinline__ T les_inv_tau(const T *f, T rho_m, T rho, T jx, T jy,
     T pxy = (f[4] + f[5]) - (f[6] + f[7]);
     T pxz = (f[8] + f[9]) - (f[10] + f[11]);
     T pyz = (f[12] + f[13]) - (f[14] + f[15]);
-    T r3 = rho_m * LC<T>::w0;
+    T r3 = rho * LC<T>::w0;
     T qxx = (pxx - r3) - jx * jx;
     T qyy = (pyy - r3) - jy * jy;
     T qzz = (pzz - r3) - jz * jz;
@@ -400,10 +398,10 @@ __device__ __forceinline__ T alpha_ref_forcing(T *f, T rho, T gx, T gy, T gz) {
 // FLUID collision shared by both kernels: optional LES rate, then BGK
 // f_i += w (feq_i - f_i) (src/cl_programs/lbm_alpha.cl:185-300).
 template <typename T>
-__device__ __forceinline__ void collide_fluid(T *f, T rho, T rho_full, T vx, T vy, T vz, T dd_param, T inv_tau,
-                                              T tau0, T les_coeff) {
+__device__ __forceinline__ void collide_fluid(T *f, T rho, T vx, T vy, T vz, T dd_param, T inv_tau, T tau0,
+                                              T les_coeff) {
     T w = inv_tau;
-    if (les_coeff > T(0)) w = les_inv_tau(f, rho, rho_full, vx, vy, vz, tau0, les_coeff);
+    if (les_coeff > T(0)) w = les_inv_tau(f, rho, vx, vy, vz, tau0, les_coeff);
     T q[19];
     feq19(dd_param, vx, vy, vz, q);
 #pragma unroll
@@ -428,18 +426,122 @@ __device__ __forceinline__ int wrap_idx(int a, int cells) {
 }
 
 // ---- the step kernel -------------------------------------------------------
+// The two-byte storages (f[] holds f_i - w_i in FP32): the update of one cell
+// from the sums s_k = f_a + f_b and differences d_k = f_a - f_b of the nine
+// opposite pairs (a, b) = (2k, 2k + 1), k over the directions
+//   x, y, x+y, x-y, x+z, x-z, y+z, y-z, z.
+// rho, j and the Smagorinsky stress come from them with 48 additions instead
+// of 81.  The equilibrium, the relaxation and the body force of a pair along
+// v = e_a . j take one product, one fused product-sum and two sums each:
+//   P = omega w (dd + 4.5 v^2),  Q = 3 omega w v + c F_v,
+//   f_a <- (1 - omega) f_a + (P + Q),  f_b <- (1 - omega) f_b + (P - Q),
+// which is f += omega (f_eq - f), then f += the force term, regrouped.  In
+// shifted values rho sums to rho - 1, j is unchanged (sum w_i e_i = 0), the
+// equilibrium from dd = (rho - 1) - 1.5 |j|^2 is f_eq - w, and the stress loses
+// the w terms exactly (sum e_a e_b w_i = delta_ab / 3 cancels rho / 3); the
+// full density enters only the LES divisor, the forcing and the density
+// output.  The LES
+// rate uses the hardware square root and reciprocal (1 ulp in FP32), far below
+// the storage's rounding.  Only these storages take this form: the FP32 and
+// FP64 kernels keep the reference's expression order, which their bit parity
+// with the CPU oracle rests on.
+template <bool BETA>
+__device__ __forceinline__ bool cell_update_pairs(float *f, int flag, const DomDev *D, float &vx, float &vy,
+                                                  float &vz, float &dens_out) {
+    float s[9], d[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        s[k] = f[2 * k] + f[2 * k + 1];
+        d[k] = f[2 * k] - f[2 * k + 1];
+    }
+    const float s23 = s[2] + s[3], s45 = s[4] + s[5], s67 = s[6] + s[7];
+    const float rho = ((s[0] + s[1]) + (s[8] + f[18])) + ((s23 + s45) + s67);  // rho - 1
+    vx = ((d[0] + d[2]) + (d[3] + d[4])) + d[5];
+    vy = ((d[1] + d[2]) - (d[3] - d[6])) + d[7];
+    vz = ((d[8] + d[4]) - (d[5] - d[6])) - d[7];
+    const float rho_full = 1.0f + rho;
+    dens_out = rho_full;
+    if (flag == kFlagFluid) {
+        const float ddp = rho - 1.5f * vel2_of(vx, vy, vz);
+        float om = (float)D->inv_tau;
+        const float les = (float)D->les_coeff;
+        if (les > 0.0f) {
+            const float r3 = rho * LC<float>::w0;
+            const float qxx = ((s[0] + s23) + (s45 - r3)) - vx * vx;
+            const float qyy = ((s[1] + s23) + (s67 - r3)) - vy * vy;
+            const float qzz = ((s[8] + s45) + (s67 - r3)) - vz * vz;
+            const float qxy = (s[2] - s[3]) - vx * vy;
+            const float qxz = (s[4] - s[5]) - vx * vz;
+            const float qyz = (s[6] - s[7]) - vy * vz;
+            const float diag = (qxx * qxx + qyy * qyy) + qzz * qzz;
+            const float off = (qxy * qxy + qxz * qxz) + qyz * qyz;
+            const float q = __builtin_amdgcn_sqrtf(diag + 2.0f * off);
+            const float tau0 = (float)D->tau0;
+            const float t = (les * q) * __builtin_amdgcn_rcpf(rho_full);
+            const float root = __builtin_amdgcn_sqrtf(tau0 * tau0 + t);
+            om = 2.0f * __builtin_amdgcn_rcpf(tau0 + root);
+        }
+        const float om1 = 1.0f - om;
+        const float o1 = om * LC<float>::w1, o2 = om * LC<float>::w2;
+        const float a1 = o1 * ddp, a2 = o2 * ddp, b1 = 4.5f * o1, b2 = 4.5f * o2, c1 = 3.0f * o1, c2 = 3.0f * o2;
+        const float v[9] = {vx, vy, vx + vy, vx - vy, vx + vz, vx - vz, vy + vz, vy - vz, vz};
+        float qv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) qv[k] = (k < 2 || k == 8 ? c1 : c2) * v[k];
+        const bool body = D->forcing == 1;
+        if (body) {
+            const float fx = rho_full * (float)D->gx, fy = rho_full * (float)D->gy, fz = rho_full * (float)D->gz;
+            const float fv[9] = {fx, fy, fx + fy, fx - fy, fx + fz, fx - fz, fy + fz, fy - fz, fz};
+#pragma unroll
+            for (int k = 0; k < 9; ++k) qv[k] += (k < 2 || k == 8 ? LC<float>::c1 : LC<float>::c2) * fv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const bool axis = k < 2 || k == 8;
+            const float pv = (axis ? b1 : b2) * (v[k] * v[k]) + (axis ? a1 : a2);
+            f[2 * k] = om1 * f[2 * k] + (pv + qv[k]);
+            f[2 * k + 1] = om1 * f[2 * k + 1] + (pv - qv[k]);
+        }
+        f[18] = om1 * f[18] + (om * LC<float>::w0) * ddp;
+        if (!body) {
+            if (!BETA) dens_out = alpha_ref_forcing(f, rho_full, (float)D->gx, (float)D->gy, (float)D->gz);
+            else dens_out = 1.0f + ddp;  // `#define dd_param rho` (lbm_beta.cl:493)
+        }
+        return true;
+    }
+    if (flag == kFlagObstacle) {
+        vx = 0.0f; vy = 0.0f; vz = 0.0f;
+        if (BETA) {  // lbm_beta.cl:570-578
+#pragma unroll
+            for (int i = 0; i < 18; i += 2) {
+                const float tt = f[i + 1];
+                f[i + 1] = f[i];
+                f[i] = tt;
+            }
+        }
+        return BETA;
+    }
+    if (flag == kFlagInjection) {  // lbm_alpha.cl:345-480, lbm_beta.cl:582-653
+        vx = (float)D->lid; vy = 0.0f; vz = 0.0f;
+        const float dd_param = -1.5f * vel2_of(vx, vy, vz);
+        feq19(dd_param, vx, vy, vz, f);
+        if (BETA) dens_out = 1.0f + dd_param;
+        else if (D->forcing == 0) dens_out = alpha_ref_forcing(f, 1.0f, (float)D->gx, (float)D->gy, (float)D->gz);
+        else dens_out = 1.0f;
+        return true;
+    }
+    return BETA;
+}
+
 // Moments in the kernel's summation order, the flag branch and the collision
 // of one cell.  f[] holds the gathered (beta) or local (alpha) populations on
 // entry and the values to store on exit; returns whether they are stored
 // (alpha stores FLUID and VELOCITY_INJECTION cells only, beta every cell).
-//
-// SHIFT: f[] holds f_i - w_i (shifted storage).  Then `rho` below sums to
-// rho - 1, j is unchanged (sum w_i e_i = 0), the equilibrium from
-// dd_param - 1 is f_eq - w, and the Smagorinsky stress loses the w terms
-// exactly (sum e_a e_b w_i = delta_ab / 3 cancels rho / 3); the full density
-// enters only the LES divisor, the forcing and the density output.
+// SHIFT (the two-byte storages, f[] holding f_i - w_i) takes the pair form
+// above.
 template <typename T, bool BETA, bool SHIFT = false>
 __device__ __forceinline__ bool cell_update(T *f, int flag, const DomDev *D, T &vx, T &vy, T &vz, T &dens_out) {
+    if constexpr (SHIFT) return cell_update_pairs<BETA>(f, flag, D, vx, vy, vz, dens_out);
     T rho;
     if (BETA) {  // summation order of lbm_beta.cl:268-483
         rho = f[3]; vy = -f[3];
@@ -482,18 +584,18 @@ __device__ __forceinline__ bool cell_update(T *f, int flag, const DomDev *D, T &
         rho = rho + f[17]; vz = vz - f[17];
         rho = rho + f[18];
     }
-    const T rho_full = SHIFT ? T(1) + rho : rho;
+    const T rho_full = rho;
     dens_out = rho_full;
     bool write_dd = BETA;
     if (flag == kFlagFluid) {
         const T dd_param = rho - T(1.5) * vel2_of(vx, vy, vz);
-        collide_fluid(f, rho, rho_full, vx, vy, vz, dd_param, (T)D->inv_tau, (T)D->tau0, (T)D->les_coeff);
+        collide_fluid(f, rho, vx, vy, vz, dd_param, (T)D->inv_tau, (T)D->tau0, (T)D->les_coeff);
         if (D->forcing == 1) {
             body_force(f, rho_full, (T)D->gx, (T)D->gy, (T)D->gz);
         } else if (!BETA) {
             dens_out = alpha_ref_forcing(f, rho_full, (T)D->gx, (T)D->gy, (T)D->gz);
         } else {
-            dens_out = SHIFT ? T(1) + dd_param : dd_param;  // `#define dd_param rho` (lbm_beta.cl:493)
+            dens_out = dd_param;  // `#define dd_param rho` (lbm_beta.cl:493)
         }
         write_dd = true;
     } else if (flag == kFlagObstacle) {
@@ -509,9 +611,9 @@ __device__ __forceinline__ bool cell_update(T *f, int flag, const DomDev *D, T &
     } else if (flag == kFlagInjection) {  // lbm_alpha.cl:345-480, lbm_beta.cl:582-653
         vx = (T)D->lid; vy = T(0); vz = T(0);
         const T one = T(1);
-        const T dd_param = SHIFT ? T(-1.5) * vel2_of(vx, vy, vz) : one - T(1.5) * vel2_of(vx, vy, vz);
+        const T dd_param = one - T(1.5) * vel2_of(vx, vy, vz);
         feq19(dd_param, vx, vy, vz, f);
-        if (BETA) dens_out = SHIFT ? one + dd_param : dd_param;
+        if (BETA) dens_out = dd_param;
         else if (D->forcing == 0) dens_out = alpha_ref_forcing(f, one, (T)D->gx, (T)D->gy, (T)D->gz);
         else dens_out = one;
         write_dd = true;
