@@ -88,9 +88,11 @@ typedef struct lbm_solver_config {
     int work_group_size;    /* threads per work-group of the step kernels: 64..512, whole wavefronts; 0 = 256 */
     int device;             /* HIP device ordinal */
     int kernel_variant;     /* tuning bits: 1 non-temporal loads, 2 non-temporal stores, 4 no XCD tile map,
-                               8 / 16 two / four x-consecutive cells per work-item; supported: 0-3, 8, 11
-                               (two cells with the non-temporal policy; two-byte storages keep the pair
-                               packed in one 32-bit word) and 16, each optionally | 4 */
+                               8 / 16 two / four x-consecutive cells per work-item, 32 the XCD tile map in
+                               chunks of 16 consecutive tiles per XCD in turn (instead of one contiguous
+                               eighth of the tiles per XCD); supported: 0-3, 8, 11 (two cells with the
+                               non-temporal policy; two-byte storages keep the pair packed in one 32-bit
+                               word) and 16, each optionally | 4 or | 32 (not both) */
     int lds_per_block;      /* bytes of LDS reserved per work-group (caps residency); 0 = none */
 } lbm_solver_config;
 
@@ -279,9 +281,9 @@ int lbm_group_get_alpha_work_group(const lbm_group *g, int *work_group_size);
  * (lbm_group_set_alpha_work_group: the rows per member of its chain tiles)
  * and keeps alpha's own size if it is faster (read it back with
  * lbm_group_get_alpha_work_group; a size set before is replaced); last, the
- * chosen shape with the XCD tile map (variant bit 4) switched.  The timed
- * steps are real steps of the simulation (40 * (pairs + 1) of them, plus one
- * if the counter was odd) */
+ * chosen shape with the two other XCD tile maps (eighths, chunks, none:
+ * variant bits 32 and 4).  The timed steps are real steps of the simulation
+ * (42 * (pairs + 1) of them, plus one if the counter was odd) */
 int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *best_variant, double *best_ms_per_step);
 /* the same averages split into beta and alpha step kernels */
 int lbm_group_get_timing_detail(lbm_group *g, double *beta_ms, double *alpha_ms, double *exchange_ms);

@@ -41,8 +41,10 @@ def test_version_and_defaults():
     ("bc", (1, 1, 1, 1, 1, 300), _capi.LBM_ERR_INVALID_ARGUMENT),
     ("storage", 9, _capi.LBM_ERR_INVALID_ARGUMENT),
     ("work_group_size", 100, _capi.LBM_ERR_INVALID_ARGUMENT),
-    ("kernel_variant", 10, _capi.LBM_ERR_INVALID_ARGUMENT),   # variants: 0-3, 8, 11, 16 (each | 4)
+    ("kernel_variant", 10, _capi.LBM_ERR_INVALID_ARGUMENT),   # variants: 0-3, 8, 11, 16 (each | 4 or | 32)
     ("kernel_variant", 13, _capi.LBM_ERR_INVALID_ARGUMENT),   # 9 | 4
+    ("kernel_variant", 39, _capi.LBM_ERR_INVALID_ARGUMENT),   # 3 | 4 | 32: no map and chunks
+    ("kernel_variant", 64, _capi.LBM_ERR_INVALID_ARGUMENT),
     ("kernel_variant", 9, _capi.LBM_ERR_INVALID_ARGUMENT),
     ("kernel_variant", 24, _capi.LBM_ERR_INVALID_ARGUMENT),
     ("kernel_variant", -1, _capi.LBM_ERR_INVALID_ARGUMENT),
@@ -101,12 +103,12 @@ def _variant_accepted_by(lib, variant, wg=256):
 def test_solver_group_and_server_accept_the_same_launch_shapes():
     lib = _capi.load_library()
     accepted = []
-    for v in range(-1, 33):
+    for v in range(-1, 66):
         s, g, srv = _variant_accepted_by(lib, v)
         assert s == g == srv, f"variant {v}: solver {s}, group {g}, server {srv}"
         if s:
             accepted.append(v)
-    assert accepted == [0, 1, 2, 3, 4, 5, 6, 7, 8, 11, 12, 15, 16, 20]
+    assert accepted == [0, 1, 2, 3, 4, 5, 6, 7, 8, 11, 12, 15, 16, 20, 32, 33, 34, 35, 40, 43, 48]
     for wg in (32, 64, 100, 128, 192, 256, 320, 384, 448, 512, 576, 1024):
         s, g, srv = _variant_accepted_by(lib, 11, wg)
         assert s == g == srv == (wg in (64, 128, 192, 256, 320, 384, 448, 512))

@@ -4,7 +4,7 @@ Seeded random groups (subdomain sizes, decompositions, periodic axes,
 storage, exchange mode, a frozen member) are stepped twice from the same
 state: once with one plain shape (work-group 256, kernel variant 0, x faces by
 the copy launch, graph replay) and once with a random shape (work-group size,
-kernel variant with its cache policy, cells per lane and tile map, alpha's own
+kernel variant with its cache policy, cells per lane and XCD tile map, alpha's own
 work-group size, x faces inside the kernel or by copies, graph or eager
 launches, subdomains in one allocation or not; a fifth of the cases are
 periodic rings of one in packed pairs, which take the seam wavefronts).
@@ -24,6 +24,7 @@ from tests.test_gpu_parity import build_gpu_group, gpu_dd, say  # noqa: E402
 NUMS = [(1, 1, 1), (2, 1, 1), (3, 1, 1), (2, 2, 1), (1, 2, 2), (2, 1, 2), (2, 2, 2), (4, 1, 1)]
 STORAGES = [_capi.STORAGE_F32, _capi.STORAGE_F64, _capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED]
 VARIANTS = [0, 1, 2, 3, 8, 11, 16]
+XCD_MAPS = [0, 0, 4, 32]   # eighths (twice as likely), none, chunks
 WGS = [64, 128, 256, 512]
 
 
@@ -41,7 +42,7 @@ def draw(case: int):
     periodic = tuple(bool(rng.random() < 0.3) for _ in range(3))
     shape = dict(
         wg=WGS[int(rng.integers(4))],
-        variant=VARIANTS[int(rng.integers(len(VARIANTS)))] | (4 if rng.random() < 0.25 else 0),
+        variant=VARIANTS[int(rng.integers(len(VARIANTS)))] | XCD_MAPS[int(rng.integers(len(XCD_MAPS)))],
         alpha_wg=[0, 64, 128, 256, 512][int(rng.integers(5))],
         fused=[0, 1, _capi.FUSED_X_AUTO][int(rng.integers(3))],
         graph=bool(rng.random() < 0.7),
@@ -55,7 +56,8 @@ def draw(case: int):
         dims = (2 * int(rng.integers(64, 100)), nums[1] * int(rng.integers(6, 16)), int(rng.integers(6, 16)))
         periodic = (True, periodic[1], periodic[2])
         storage = STORAGES[2 + int(rng.integers(2))]
-        shape.update(variant=(8, 11)[int(rng.integers(2))], fused=(1, _capi.FUSED_X_AUTO)[int(rng.integers(2))])
+        shape.update(variant=(8, 11)[int(rng.integers(2))] | XCD_MAPS[int(rng.integers(len(XCD_MAPS)))],
+                     fused=(1, _capi.FUSED_X_AUTO)[int(rng.integers(2))])
     n = nums[0] * nums[1] * nums[2]
     stop = (int(rng.integers(n)), int(rng.integers(1, 6))) if rng.random() < 0.3 else None
     return dict(dims=dims, nums=nums, periodic=periodic, storage=storage, mode=int(rng.integers(2)), shape=shape,

@@ -41,7 +41,7 @@ def draw(case: int):
                 lid=float(rng.uniform(0.0, 0.1)), lid_layer=bool(rng.random() < 0.7),
                 forcing=max(forcing, 0), g=g, storage=(_capi.STORAGE_F32, _capi.STORAGE_F64)[int(rng.integers(2))],
                 wg=(64, 128, 256, 512)[int(rng.integers(4))],
-                variant=VARIANTS[int(rng.integers(len(VARIANTS)))] | (4 if rng.random() < 0.25 else 0),
+                variant=VARIANTS[int(rng.integers(len(VARIANTS)))] | (0, 0, 4, 32)[int(rng.integers(4))],
                 amp=float(rng.choice([0.0, 1e-3, 1e-2])), seed=int(rng.integers(1 << 30)),
                 steps=int(rng.integers(1, 10)), box=box)
 
@@ -85,7 +85,8 @@ def test_random_decomposition_equals_oracle(case):
     dims = tuple(nums[a] * int(rng.integers(6, 22)) for a in range(3))
     periodic = tuple(bool(rng.random() < 0.4) for _ in range(3))
     kw = dict(inv_tau=1 / 0.55, tau=0.55, smagorinsky_cs=(0.0, 0.17)[int(rng.integers(2))])
-    wg, variant = (64, 128, 256, 512)[int(rng.integers(4))], VARIANTS[int(rng.integers(len(VARIANTS)))]
+    wg = (64, 128, 256, 512)[int(rng.integers(4))]
+    variant = VARIANTS[int(rng.integers(len(VARIANTS)))] | (0, 0, 4, 32)[int(rng.integers(4))]
     alpha_wg, fused = (0, 128, 512)[int(rng.integers(3))], (0, 1)[int(rng.integers(2))]
     say(f"case {case}: dims {dims} nums {nums} periodic {periodic} cs {kw['smagorinsky_cs']} wg {wg} "
         f"variant {variant} alpha {alpha_wg} fused {fused}")

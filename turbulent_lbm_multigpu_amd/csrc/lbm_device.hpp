@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include "exchange_plan.hpp"  // kMaxChain, kChainRows
+#include "launch_shape.hpp"   // variant bits of the XCD tile map
 
 namespace lbm {
 
@@ -418,6 +419,18 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     int q = nb >> 3, r = nb & 7;
     int x = b & 7, k = b >> 3;
     return x * q + (x < r ? x : r) + k;
+}
+
+// Block -> tile map in chunks: the k-th block dispatched to XCD x takes
+// offset k % K of chunk (k / K) * 8 + x, so each XCD walks K consecutive tiles
+// while the 8 XCDs together sweep 8 K adjacent tiles.  The blocks of the
+// last incomplete round of 8 K keep their own index.  A bijection of
+// [0, nb) for any nb and K > 0.
+__device__ __forceinline__ int xcd_chunk_tile(int b, int nb, int K) {
+    const int full = nb / (8 * K) * (8 * K);
+    if (b >= full) return b;
+    const int x = b & 7, k = b >> 3;
+    return ((k / K) * 8 + x) * K + (k - (k / K) * K);
 }
 
 __device__ __forceinline__ int wrap_idx(int a, int cells) {
@@ -1289,7 +1302,10 @@ __global__ void __launch_bounds__(512) step_kernel(const DomDev *__restrict__ do
         if (blockIdx.x == 0 && threadIdx.x == 0) ctr[BETA ? 1 : 0] = h + 1;
     }
     const int nb = (int)gridDim.x;
-    const int t = xcd_map ? xcd_tile((int)blockIdx.x, nb) : (int)blockIdx.x;
+    // xcd_map: < 0 one contiguous eighth per XCD, 0 none, K > 0 chunks of K
+    const int t = xcd_map < 0   ? xcd_tile((int)blockIdx.x, nb)
+                  : xcd_map > 0 ? xcd_chunk_tile((int)blockIdx.x, nb, xcd_map)
+                                : (int)blockIdx.x;
     const int td = tile_dom[t];
     if (td < 0) {  // a tile of fused x chain -td-1 (uniform per work-group: the barrier is reached by all)
         // per row tile: the handoff tile, then the interior tiles of a one-row chain
@@ -1350,21 +1366,22 @@ __global__ void __launch_bounds__(512) step_kernel(const DomDev *__restrict__ do
 }
 
 // Launch of step_kernel for storage ST: the half-step and the variant bits
-// become template arguments; bit 2 of the variant turns the XCD-contiguous
-// tile map off; `lds` reserves LDS per work-group to cap residency.  The 4-
+// become template arguments; bit 2 of the variant turns the XCD tile map off
+// and bit 5 deals the tiles to the XCDs in chunks instead of eighths; `lds`
+// reserves LDS per work-group to cap residency.  The 4-
 // and 8-byte storages are instantiated in lbm_runtime.hip, the two-byte ones
 // in lbm_step_2byte.hip (a translation unit of their own: see there).
 template <int ST>
 hipError_t launch_step_storage(bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs, const int *tile_table,
                        hipStream_t stream, long long *ctr, const XChain *chains) {
-    const int xcd = (variant & 4) ? 0 : 1;
+    const int xcd = (variant & kVarNoXcdMap) ? 0 : (variant & kVarXcdChunks) ? kXcdChunkTiles : -1;
     auto go = [&](auto b, auto vv) {
         constexpr bool B = decltype(b)::value;
         constexpr int V = decltype(vv)::value;
         step_kernel<ST, B, V><<<tiles, wg, lds, stream>>>(descs, tile_table, chains, xcd, ctr);
     };
     auto pick = [&](auto b) {
-        switch (variant & ~4) {
+        switch (variant & ~(kVarNoXcdMap | kVarXcdChunks)) {
             case 0: go(b, std::integral_constant<int, 0>()); break;
             case 1: go(b, std::integral_constant<int, 1>()); break;
             case 2: go(b, std::integral_constant<int, 2>()); break;

@@ -673,7 +673,7 @@ static int validate_config(const lbm_solver_config &c) {
         return record(nullptr, LBM_ERR_PARAMETRIZATION, "LES needs smagorinsky_cs >= 0 and tau > 0.5");
     if (!lbm_variant_supported(c.kernel_variant) || c.lds_per_block < 0 || c.lds_per_block > 65536)
         return record(nullptr, LBM_ERR_INVALID_ARGUMENT,
-                      "kernel_variant must be one of 0, 1, 2, 3, 8, 11, 16 (optionally | 4); lds_per_block in [0, 65536]");
+                      "kernel_variant must be one of 0, 1, 2, 3, 8, 11, 16 (optionally | 4 or | 32); lds_per_block in [0, 65536]");
     return LBM_OK;
 }
 
@@ -2220,8 +2220,9 @@ extern "C" int lbm_group_get_alpha_work_group(const lbm_group *g, int *work_grou
 // policy) and keeps the fastest; then, with that shape for beta, times
 // alpha at each other work-group size (which sets how many rows of each
 // member an alpha chain tile holds) and keeps alpha's own size if it is
-// faster; last, it times the chosen shape with the XCD tile map switched
-// (variant bit 4) and keeps that if it is faster.  The steps it runs are
+// faster; last, it times the chosen shape with the two other XCD tile maps
+// (one contiguous eighth of the tiles per XCD, chunks of consecutive tiles
+// dealt in turn, or none: variant bits 32 and 4) and keeps the fastest.  The steps it runs are
 // ordinary simulation steps: every launch shape computes the same
 // arithmetic, so the state is the one the same number of steps gives with
 // any shape.
@@ -2271,19 +2272,20 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
         return LBM_OK;
     };
     double best = 1e300;
+    const int map = lbm_variant_map_bits(g->variant);
     int bw = g->wg, bv = g->variant, ba = 0;
     int rc = lbm_group_set_alpha_work_group(g, 0);  // stage 1: one size for both kernels
     if (rc) return rc;
     for (int wi = 0; wi < 4; ++wi)
         for (int vi = 0; vi < 4; ++vi) {
             double per = 0.0;
-            rc = lbm_group_set_launch(g, wgs[wi], vars[vi] | (g->variant & 4));
+            rc = lbm_group_set_launch(g, wgs[wi], vars[vi] | map);
             if (!rc) rc = time_shape(&per);
             if (rc) return rc;
             if (per < best) {
                 best = per;
                 bw = wgs[wi];
-                bv = vars[vi] | (g->variant & 4);
+                bv = vars[vi] | map;
             }
         }
     rc = lbm_group_set_launch(g, bw, bv);
@@ -2301,15 +2303,23 @@ extern "C" int lbm_group_autotune(lbm_group *g, int pairs, int *best_wg, int *be
     }
     rc = lbm_group_set_alpha_work_group(g, ba);
     if (rc) return rc;
-    {  // stage 3: the XCD tile map the other way round
-        double per = 0.0;
-        rc = lbm_group_set_launch(g, bw, bv ^ 4);
-        if (!rc) rc = time_shape(&per);
-        if (rc) return rc;
-        if (per < best) {
-            best = per;
-            bv ^= 4;
+    {  // stage 3: the two other XCD tile maps
+        const int maps[3] = {0, kVarXcdChunks, kVarNoXcdMap};
+        const int base = bv & ~(kVarNoXcdMap | kVarXcdChunks);
+        int keep = bv;
+        for (int mi = 0; mi < 3; ++mi) {
+            const int v = base | maps[mi];
+            if (v == bv) continue;
+            double per = 0.0;
+            rc = lbm_group_set_launch(g, bw, v);
+            if (!rc) rc = time_shape(&per);
+            if (rc) return rc;
+            if (per < best) {
+                best = per;
+                keep = v;
+            }
         }
+        bv = keep;
     }
     rc = lbm_group_set_launch(g, bw, bv);
     if (!rc) rc = group_prepare(g);

@@ -27,8 +27,9 @@ STORAGES = {"f64": _capi.STORAGE_F64, "f32": _capi.STORAGE_F32, "f16": _capi.STO
             "bf16": _capi.STORAGE_BF16_SHIFTED}
 # fastest measured kernel variant per storage on this channel: non-temporal
 # accesses, one cell per work-item for f64/f32, two for the two-byte storages
-# (their pair of cells moves as one 32-bit word)
-DEFAULT_VARIANT = {"f64": 3, "f32": 3, "f16": 11, "bf16": 11}
+# (their pair of cells moves as one 32-bit word), tiles dealt to the XCDs in
+# chunks (| 32)
+DEFAULT_VARIANT = {"f64": 35, "f32": 35, "f16": 43, "bf16": 43}
 
 
 def build_channel(dims, storage, *, tau=0.51, cs=0.17, gx=1e-6, sphere_seed=20260101, fill=0.05, rmin=3.0,
@@ -109,7 +110,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--storages", nargs="+", default=["f64", "f32", "f16", "bf16"])
     ap.add_argument("--variant", type=int, default=None,
-                    help="kernel variant for every storage (default: 3 for f64/f32, 11 for f16/bf16)")
+                    help="kernel variant for every storage (default: 35 for f64/f32, 43 for f16/bf16)")
     ap.add_argument("--fused-x", type=int, default=_capi.FUSED_X_AUTO, choices=[0, 1, _capi.FUSED_X_AUTO],
                     help="periodic x faces by a copy launch (0), inside the step kernel (1), or per "
                          "launch shape (2: run tiles for one cell per lane, seam wavefronts for packed "
