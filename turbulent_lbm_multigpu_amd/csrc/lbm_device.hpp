@@ -453,11 +453,10 @@ __device__ __forceinline__ int wrap_idx(int a, int cells) {
 // equilibrium from dd = (rho - 1) - 1.5 |j|^2 is f_eq - w, and the stress loses
 // the w terms exactly (sum e_a e_b w_i = delta_ab / 3 cancels rho / 3); the
 // full density enters only the LES divisor, the forcing and the density
-// output.  The LES
-// rate uses the hardware square root and reciprocal (1 ulp in FP32), far below
-// the storage's rounding.  Only these storages take this form: the FP32 and
-// FP64 kernels keep the reference's expression order, which their bit parity
-// with the CPU oracle rests on.
+// output.  The LES rate uses the hardware square root and reciprocal (1 ulp in
+// FP32), far below the storage's rounding.  Only these storages take this
+// form: the FP32 and FP64 kernels keep the reference's expression order, which
+// their bit parity with the CPU oracle rests on.
 template <bool BETA>
 __device__ __forceinline__ bool cell_update_pairs(float *f, int flag, const DomDev *D, float &vx, float &vy,
                                                   float &vz, float &dens_out) {
