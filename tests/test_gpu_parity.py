@@ -432,6 +432,50 @@ def test_fused_x_auto_picks_per_storage(storage, dims, variant):
 
 
 @pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
+@pytest.mark.parametrize("kind", ["bgk", "les", "les_body_force", "les_reference_forcing"])
+@pytest.mark.parametrize("variant", [0, 11])
+def test_two_byte_half_steps_equal_oracle_to_storage_rounding(storage, kind, variant):
+    """The two-byte storages collide in their own form (pair sums and
+    differences, the hardware square root and reciprocal in the LES rate),
+    so they are held to the exact update rather than to FP32's rounding
+    order: from the same state, exactly representable in the storage, one
+    beta and then one alpha half-step must give every slot of the FP64
+    oracle to within half a storage ulp of f - w_i (the output rounding)
+    + 1e-7 (FP32 arithmetic on f - w_i), and the velocity and density
+    outputs (FP32, no storage rounding) to within 2e-7.  (The FP32 oracle
+    itself is up to 2.4e-7 from FP64 here: it collides the full f, whose
+    LES stress cancels the w terms in FP32.)  A dropped term, a wrong sign or
+    index in the pair algebra, or a wrong LES constant moves a slot by 1e-5
+    or more at these amplitudes (FP16's half ulp at |f - w| = 0.03 is
+    7.6e-6; tests/test_two_byte_tolerance_cpu.py plants such mistakes in the
+    oracle and checks that they exceed this tolerance)."""
+    from tests.test_two_byte_tolerance_cpu import W19, oracle_domain, tolerance
+
+    cfg = {"bgk": dict(tau=0.6, cs=0.0, forcing=1, g=(0.0, 0.0, 0.0)),
+           "les": dict(tau=0.505, cs=0.17, forcing=1, g=(0.0, 0.0, 0.0)),
+           "les_body_force": dict(tau=0.505, cs=0.17, forcing=1, g=(2e-4, -1e-4, 1.5e-4)),
+           "les_reference_forcing": dict(tau=0.505, cs=0.17, forcing=0, g=(2e-4, -1e-4, 1.5e-4))}[kind]
+    say(f"case: two-byte half-steps vs oracle, storage {storage} {kind} variant {variant}")
+    fmt = "f16" if storage == _capi.STORAGE_F16_SHIFTED else "bf16"
+    _, s = make_pair((20, 16, 14), inv_tau=1 / cfg["tau"], tau=cfg["tau"], cs=cfg["cs"], lid=0.08, g=cfg["g"],
+                     forcing=cfg["forcing"], storage=storage, amp=2e-2, seed=11, variant=variant)
+    o = oracle_domain(np.float64, **cfg)
+    o.dd[...] = gpu_dd(s)   # the oracle starts from the device's rounded state
+    for half in ("beta", "alpha"):
+        o.step()
+        s.run(1)
+        got, want = gpu_dd(s).astype(np.float64), o.dd
+        tol = tolerance(got, want, fmt)
+        err = np.abs(got - want)
+        say(f"  {half}: max |slot error| {err.max():.3e}, max error / tolerance {(err / tol).max():.3f}")
+        assert (err <= tol).all(), f"{half}: {np.count_nonzero(err > tol)} slots beyond the storage rounding"
+        np.testing.assert_allclose(s.store(_capi.FIELD_VELOCITY).reshape(3, -1), o.vel, rtol=0, atol=2e-7)
+        np.testing.assert_allclose(s.store(_capi.FIELD_DENSITY).ravel(), o.dens, rtol=0, atol=2e-7)
+        o.dd[...] = got   # continue from the device's rounded state
+    s.close()
+
+
+@pytest.mark.parametrize("storage", [_capi.STORAGE_F16_SHIFTED, _capi.STORAGE_BF16_SHIFTED])
 def test_shifted_storage_error_vs_fp64(storage):
     """FP16/BF16 storage of f - w_i with FP32 collision stays close to FP64."""
     say(f"case: shifted storage {storage} vs fp64")
