@@ -89,8 +89,8 @@ typedef struct lbm_solver_config {
     int device;             /* HIP device ordinal */
     int kernel_variant;     /* tuning bits: 1 non-temporal loads, 2 non-temporal stores, 4 no XCD tile map,
                                8 / 16 two / four x-consecutive cells per work-item, 32 the XCD tile map in
-                               chunks of 16 consecutive tiles per XCD in turn (instead of one contiguous
-                               eighth of the tiles per XCD); supported: 0-3, 8, 11 (two cells with the
+                               chunks of about 16 consecutive tiles per XCD in turn, matched to the tiles
+                               of a z plane (instead of one contiguous eighth of the tiles per XCD); supported: 0-3, 8, 11 (two cells with the
                                non-temporal policy; two-byte storages keep the pair packed in one 32-bit
                                word) and 16, each optionally | 4 or | 32 (not both) */
     int lds_per_block;      /* bytes of LDS reserved per work-group (caps residency); 0 = none */

@@ -5,8 +5,9 @@
 
 // kernel_variant bits: 1 non-temporal loads, 2 non-temporal stores, 4 no XCD
 // tile map, 8 / 16 two / four x-consecutive cells per work-item, 32 the XCD
-// tile map in chunks (kXcdChunkTiles consecutive tiles per XCD in turn)
-// instead of one contiguous eighth per XCD.  Built: 0-3, 8, 11 (two cells
+// tile map in chunks (about kXcdChunkTiles consecutive tiles per XCD in turn;
+// a group matches the count to its members' z planes) instead of one
+// contiguous eighth per XCD.  Built: 0-3, 8, 11 (two cells
 // with the non-temporal policy; the two-byte storages keep each pair packed
 // in one 32-bit word) and 16, each optionally | 4 or | 32 (not both).
 constexpr int kVarNoXcdMap = 4;

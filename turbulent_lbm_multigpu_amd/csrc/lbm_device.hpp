@@ -1366,14 +1366,14 @@ __global__ void __launch_bounds__(512) step_kernel(const DomDev *__restrict__ do
 
 // Launch of step_kernel for storage ST: the half-step and the variant bits
 // become template arguments; bit 2 of the variant turns the XCD tile map off
-// and bit 5 deals the tiles to the XCDs in chunks instead of eighths; `lds`
-// reserves LDS per work-group to cap residency.  The 4-
+// and bit 5 deals the tiles to the XCDs in chunks of `xcd_chunk` tiles
+// instead of eighths; `lds` reserves LDS per work-group to cap residency.  The 4-
 // and 8-byte storages are instantiated in lbm_runtime.hip, the two-byte ones
 // in lbm_step_2byte.hip (a translation unit of their own: see there).
 template <int ST>
 hipError_t launch_step_storage(bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs, const int *tile_table,
-                       hipStream_t stream, long long *ctr, const XChain *chains) {
-    const int xcd = (variant & kVarNoXcdMap) ? 0 : (variant & kVarXcdChunks) ? kXcdChunkTiles : -1;
+                       hipStream_t stream, long long *ctr, const XChain *chains, int xcd_chunk = kXcdChunkTiles) {
+    const int xcd = (variant & kVarNoXcdMap) ? 0 : (variant & kVarXcdChunks) ? xcd_chunk : -1;
     auto go = [&](auto b, auto vv) {
         constexpr bool B = decltype(b)::value;
         constexpr int V = decltype(vv)::value;
@@ -1398,6 +1398,7 @@ hipError_t launch_step_storage(bool beta, int variant, int lds, int tiles, int w
 
 // launch_step_storage for the FP16 / BF16 storages (2, 3), defined in lbm_step_2byte.hip
 hipError_t launch_step_2byte(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
-                             const int *tile_table, hipStream_t stream, long long *ctr, const XChain *chains);
+                             const int *tile_table, hipStream_t stream, long long *ctr, const XChain *chains,
+                             int xcd_chunk);
 
 }  // namespace lbm

@@ -401,7 +401,22 @@ struct LaunchTables {
     int ntiles = 0;
     XChain *d_chains = nullptr;
     size_t chains_cap = 0;
+    int xcd_chunk = kXcdChunkTiles;  // tiles per XCD chunk of the chunked tile map
 };
+
+// Tiles per XCD chunk for a launch whose members all have `plane` tiles per z
+// plane (0: unknown or not uniform).  Beta gathers from the z neighbours,
+// `plane` tiles away; when 8 chunks of K tiles divide the plane, the tile a
+// plane ahead falls to the same XCD a round or more later, its rows still in
+// that XCD's L2.  Keeps kXcdChunkTiles when that already holds or nothing in
+// [12, 24] does.
+static int xcd_chunk_for_plane(long long plane) {
+    if (plane <= 0 || plane % (8 * kXcdChunkTiles) == 0) return kXcdChunkTiles;
+    for (int d = 1; d <= 8; ++d)
+        for (int K : {kXcdChunkTiles + d, kXcdChunkTiles - d})
+            if (K >= 12 && K <= 24 && plane % (8 * K) == 0) return K;
+    return kXcdChunkTiles;
+}
 
 struct lbm_group {
     std::vector<lbm_solver *> members;
@@ -519,14 +534,17 @@ int upload_desc(lbm_solver *s) {
 // lbm_step_2byte.hip.
 hipError_t launch_step_kernel(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
                               const int *tile_table, hipStream_t stream, long long *ctr = nullptr,
-                              const XChain *chains = nullptr) {
+                              const XChain *chains = nullptr, int xcd_chunk = kXcdChunkTiles) {
     switch (storage) {
         case LBM_STORAGE_F32:
-            return launch_step_storage<0>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
+            return launch_step_storage<0>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains,
+                                          xcd_chunk);
         case LBM_STORAGE_F64:
-            return launch_step_storage<1>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
+            return launch_step_storage<1>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains,
+                                          xcd_chunk);
         default:
-            return launch_step_2byte(storage, beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
+            return launch_step_2byte(storage, beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains,
+                                     xcd_chunk);
     }
 }
 
@@ -1486,9 +1504,13 @@ static int build_tables(lbm_group *g, int wg, LaunchTables &T) {
         if (i < g->xfuse.size())
             for (int side = 0; side < 2; ++side) descs[i].xmask_a[side] = g->xfuse[i].mask[side];
     }
+    long long plane = -1;  // tiles per z plane, shared by every member so far (0: not shared or not whole)
+    auto note_plane = [&plane](long long p) { plane = (plane < 0 || plane == p) ? p : 0; };
     for (size_t i = 0; i < nm; ++i) {
         const int c = chain_of[i];
         if (c < 0) {
+            const long long pc = (long long)g->members[i]->cfg.dims[0] * g->members[i]->cfg.dims[1];
+            note_plane(pc % per_tile == 0 ? pc / per_tile : 0);
             const int nt = (int)((g->members[i]->cells + per_tile - 1) / per_tile);
             descs[i].tile_begin = (int)tiles.size();
             descs[i].ntiles = nt;
@@ -1501,8 +1523,9 @@ static int build_tables(lbm_group *g, int wg, LaunchTables &T) {
             // a periodic ring of one in packed pairs: its seam stays in one
             // wavefront per row (seam_pairs), tiles of whole wavefronts
             const int X = g->members[i]->cfg.dims[0];
-            const long long waves =
-                (long long)seam_waves_per_row(X) * g->members[i]->cfg.dims[1] * g->members[i]->cfg.dims[2];
+            const long long pw = (long long)seam_waves_per_row(X) * g->members[i]->cfg.dims[1];
+            note_plane(pw % (wg / 64) == 0 ? pw / (wg / 64) : 0);
+            const long long waves = pw * g->members[i]->cfg.dims[2];
             const int nt = (int)((waves + wg / 64 - 1) / (wg / 64));
             descs[i].tile_begin = (int)tiles.size();
             descs[i].ntiles = nt;
@@ -1537,6 +1560,8 @@ static int build_tables(lbm_group *g, int wg, LaunchTables &T) {
             xc.right[k] = g->xfuse[mem].nb[1] >= 0 ? pos[g->xfuse[mem].nb[1]] : -1;
         }
         const int nt = (int)((rows + R - 1) / R) * (1 + xc.ipasses);
+        const long long Y = m0->cfg.dims[1];
+        note_plane(Y % R == 0 ? Y / R * (1 + xc.ipasses) : 0);
         for (int k = 0; k < M; ++k) {
             descs[ch[k]].tile_begin = xc.tile_begin;
             descs[ch[k]].ntiles = nt;
@@ -1548,6 +1573,7 @@ static int build_tables(lbm_group *g, int wg, LaunchTables &T) {
     GCHK(grow_table(T.d_descs, T.descs_cap, descs.size()));
     if (!xch.empty()) GCHK(grow_table(T.d_chains, T.chains_cap, xch.size()));
     T.ntiles = (int)tiles.size();
+    T.xcd_chunk = xcd_chunk_for_plane(plane > 0 ? plane : 0);
     if (!tiles.empty())
         GCHK(hipMemcpyAsync(T.d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, g->stream));
     if (!xch.empty())
@@ -2020,7 +2046,7 @@ static int group_launch_half(lbm_group *g, bool beta, bool timed) {
         GCHK(hipEventRecord(ea, g->stream));
     }
     GCHK(launch_step_kernel(g->storage, beta, g->variant, g->lds, T.ntiles, split ? g->wg_alpha : g->wg, T.d_descs,
-                            T.d_tiles, g->stream, g->d_ctr, T.d_chains));
+                            T.d_tiles, g->stream, g->d_ctr, T.d_chains, T.xcd_chunk));
     if (timed) GCHK(hipEventRecord(eb, g->stream));
     // the kernel just run was beta -> syncBeta, alpha -> syncAlpha (src/CController.hpp:385-391)
     const int k = beta ? 1 : 0;

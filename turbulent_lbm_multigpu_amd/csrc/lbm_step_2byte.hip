@@ -21,10 +21,11 @@
 namespace lbm {
 
 hipError_t launch_step_2byte(int storage, bool beta, int variant, int lds, int tiles, int wg, const DomDev *descs,
-                             const int *tile_table, hipStream_t stream, long long *ctr, const XChain *chains) {
+                             const int *tile_table, hipStream_t stream, long long *ctr, const XChain *chains,
+                             int xcd_chunk) {
     if (storage == LBM_STORAGE_F16_SHIFTED)
-        return launch_step_storage<2>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
-    return launch_step_storage<3>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains);
+        return launch_step_storage<2>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains, xcd_chunk);
+    return launch_step_storage<3>(beta, variant, lds, tiles, wg, descs, tile_table, stream, ctr, chains, xcd_chunk);
 }
 
 }  // namespace lbm
